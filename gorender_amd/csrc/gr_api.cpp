@@ -1,0 +1,1007 @@
+// C ABI of the render path: context, resident meshes/textures in HBM, and the
+// per-draw orchestration of the five kernels on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gorender_hip.h"
+#include "gr_device.h"
+#include "gr_launch.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+gr_status fail(gr_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define GR_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(e_ == hipErrorOutOfMemory ? GR_ERR_OUT_OF_MEMORY : GR_ERR_DEVICE, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct MeshRes {
+    bool live = false;
+    gr::DevMesh dm{};
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    bool ni_valid = true;
+    std::vector<int32_t> textures;  // distinct texture handles referenced by faces
+};
+
+struct TexRes {
+    bool live = false;
+    gr::DevTexture dt{};
+    void* texels = nullptr;
+    int64_t bytes = 0;
+};
+
+struct Staging {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+};
+
+struct TimedLaunch {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct gr_context {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<MeshRes> meshes;
+    std::vector<TexRes> textures;
+    DevBuf d_textures;
+    bool tex_dirty = true;
+
+    DevBuf objs, frames, clip, nint, vis, rrec, srec, wrec, tiles, zero32, zero64, wgcount,
+        refs, out_rgba, out_depth, ctrl;
+    Staging ring[4];
+    int ring_i = 0;
+    void* h_stats = nullptr;  // pinned: stats of the last batch
+    size_t h_stats_cap = 0;
+    void* h_ctrl = nullptr;   // pinned: sticky control words
+    int32_t last_frames = 0;
+
+    gr_config cfg{32, 16, 1 << 22, 4, 0};
+
+    bool timing = false;
+    std::vector<TimedLaunch> timed;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> kernel_ms;
+};
+
+namespace {
+
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) r++;
+    return r;
+}
+
+gr_status ensure(gr_context* c, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (b.cap >= bytes) return GR_OK;
+    if (b.p) {
+        GR_HIP(hipStreamSynchronize(c->stream));
+        GR_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t cap = std::max(bytes, b.cap + b.cap / 2);
+    cap = (cap + 4095) & ~size_t(4095);
+    GR_HIP(hipMalloc(&b.p, cap));
+    b.cap = cap;
+    return GR_OK;
+}
+
+gr_status staging_get(gr_context* c, size_t bytes, void** out, int* slot) {
+    int i = c->ring_i;
+    c->ring_i = (c->ring_i + 1) % 4;
+    Staging& s = c->ring[i];
+    if (s.pending) {
+        GR_HIP(hipEventSynchronize(s.ev));
+        s.pending = false;
+    }
+    if (s.cap < bytes) {
+        if (s.host) GR_HIP(hipHostFree(s.host));
+        s.host = nullptr;
+        size_t cap = std::max(bytes, size_t(1) << 16);
+        GR_HIP(hipHostMalloc(&s.host, cap, hipHostMallocDefault));
+        s.cap = cap;
+    }
+    if (!s.ev) GR_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    *out = s.host;
+    *slot = i;
+    return GR_OK;
+}
+
+gr_status staging_mark(gr_context* c, int slot) {
+    GR_HIP(hipEventRecord(c->ring[slot].ev, c->stream));
+    c->ring[slot].pending = true;
+    return GR_OK;
+}
+
+gr_status upload(gr_context* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return GR_OK;
+    void* h;
+    int slot;
+    gr_status st = staging_get(c, bytes, &h, &slot);
+    if (st) return st;
+    std::memcpy(h, src, bytes);
+    GR_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c->stream));
+    return staging_mark(c, slot);
+}
+
+hipEvent_t pool_event(gr_context* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+template <typename F>
+void timed_launch(gr_context* c, const char* name, F f) {
+    if (!c->timing) {
+        f();
+        return;
+    }
+    hipEvent_t a = pool_event(c), b = pool_event(c);
+    if (!a || !b) {
+        f();
+        return;
+    }
+    (void)hipEventRecord(a, c->stream);
+    f();
+    (void)hipEventRecord(b, c->stream);
+    c->timed.push_back({name, a, b});
+}
+
+void collect_times(gr_context* c) {
+    for (auto& t : c->timed) {
+        float ms = 0.f;
+        if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            auto& e = c->kernel_ms[t.name];
+            e.first += ms;
+            e.second += 1;
+        }
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->timed.clear();
+}
+
+uint32_t pack_rgba(const uint8_t c[4]) {
+    return (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+}
+
+// renderer.go:50-76
+void ref_tile_bounds(int tile, int num_tiles, int w, int h, float out[4]) {
+    if (num_tiles == 1) {
+        out[0] = 0.f;
+        out[1] = 0.f;
+        out[2] = (float)w;
+        out[3] = (float)h;
+        return;
+    }
+    uint64_t ntx = (uint64_t)std::sqrt((double)num_tiles);
+    uint64_t nty = ((uint64_t)num_tiles + ntx - 1) / ntx;
+    uint64_t tw = ((uint64_t)w + ntx - 1) / ntx;
+    uint64_t th = ((uint64_t)h + nty - 1) / nty;
+    float sx = (float)(((uint64_t)tile % ntx) * tw);
+    float sy = (float)(((uint64_t)tile / ntx) * th);
+    float ex = sx + (float)tw, ey = sy + (float)th;
+    if (ex > (float)w) ex = (float)w;
+    if (ey > (float)h) ey = (float)h;
+    out[0] = sx;
+    out[1] = sy;
+    out[2] = ex;
+    out[3] = ey;
+}
+
+bool valid_tile(int v) { return v == 8 || v == 16 || v == 32 || v == 64; }
+
+gr_status validate_frame(const gr_frame& f) {
+    if (f.width < 1 || f.height < 1 || f.width > gr::kMaxDim || f.height > gr::kMaxDim)
+        return fail(GR_ERR_INVALID_ARGUMENT, "frame size must be within 1..16384");
+    if (f.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES))
+        return fail(GR_ERR_UNSUPPORTED, "edge/vertex overlays are not implemented on the GPU path");
+    if (f.ref_tiles != 1 && f.ref_tiles != 16)
+        return fail(GR_ERR_INVALID_ARGUMENT, "ref_tiles must be 1 or 16");
+    return GR_OK;
+}
+
+// Go builtin min/max semantics on floats (mesh.go:33-38).
+float go_min(float a, float b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a < b) return a;
+    if (b < a) return b;
+    return std::signbit(a) ? a : b;
+}
+float go_max(float a, float b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    if (a > b) return a;
+    if (b > a) return b;
+    return std::signbit(a) ? b : a;
+}
+
+// The whole pipeline for a batch of frames. Objects must be laid out frame by
+// frame; outputs are device pointers.
+gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
+                  int no) {
+    if (nf <= 0) return fail(GR_ERR_INVALID_ARGUMENT, "no frames");
+    const int TW = c->cfg.tile_w, TH = c->cfg.tile_h;
+    const int twl = ilog2(TW), thl = ilog2(TH);
+
+    std::vector<gr::DevFrame> frames(nf);
+    std::vector<gr::DevObject> dobj;
+    dobj.reserve(no);
+    size_t clip_n = 0, nint_n = 0;
+    int vblocks = 0, fblocks = 0, tiles = 0;
+    size_t slots = 0;
+    for (int f = 0; f < nf; f++) {
+        const gr_frame& fr = bf[f].frame;
+        gr_status st = validate_frame(fr);
+        if (st) return st;
+        if (bf[f].num_objects < 0 || bf[f].first_object < 0 ||
+            bf[f].first_object + bf[f].num_objects > no)
+            return fail(GR_ERR_INVALID_ARGUMENT, "frame object range out of bounds");
+        if (!bf[f].d_rgba || !bf[f].d_depth)
+            return fail(GR_ERR_INVALID_ARGUMENT, "null device output");
+        gr::DevFrame& d = frames[f];
+        std::memset(&d, 0, sizeof d);
+        d.W = fr.width;
+        d.H = fr.height;
+        d.flags = fr.flags;
+        d.ref_tiles = fr.ref_tiles;
+        std::memcpy(d.screen, fr.screen, sizeof d.screen);
+        std::memcpy(d.light, fr.light, sizeof d.light);
+        d.ambient = fr.ambient;
+        d.diffuse = fr.diffuse;
+        d.znear = fr.znear;
+        d.zfar = fr.zfar;
+        d.face_color = pack_rgba(fr.face_color);
+        d.clear_color = pack_rgba(fr.clear_color);
+        d.dot_color = pack_rgba(fr.dot_color);
+        d.dot_step = fr.dot_step;
+        for (int t = 0; t < fr.ref_tiles; t++)
+            ref_tile_bounds(t, fr.ref_tiles, fr.width, fr.height, d.ref_tile_rect[t]);
+        d.tiles_x = (fr.width + TW - 1) / TW;
+        d.tiles_y = (fr.height + TH - 1) / TH;
+        d.tile_begin = tiles;
+        tiles += d.tiles_x * d.tiles_y;
+        d.face_block_begin = fblocks;
+        d.rgba = (uint8_t*)bf[f].d_rgba;
+        d.depth = (float*)bf[f].d_depth;
+        int seq = 0;
+        for (int k = 0; k < bf[f].num_objects; k++) {
+            const gr_object& ob = objs[bf[f].first_object + k];
+            if (ob.mesh < 0 || ob.mesh >= (int)c->meshes.size() || !c->meshes[ob.mesh].live)
+                return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
+            const MeshRes& mr = c->meshes[ob.mesh];
+            for (int32_t th : mr.textures)
+                if (th < 0 || th >= (int)c->textures.size() || !c->textures[th].live)
+                    return fail(GR_ERR_INVALID_HANDLE, "mesh references a destroyed texture");
+            gr::DevObject o;
+            std::memset(&o, 0, sizeof o);
+            o.m = mr.dm;
+            std::memcpy(o.mvp, ob.mvp, sizeof o.mvp);
+            std::memcpy(o.world, ob.world, sizeof o.world);
+            o.frame = f;
+            o.gouraud = (fr.flags & GR_LIGHTING) && !(fr.flags & GR_FLAT_SHADING) && mr.dm.N > 0;
+            if (o.gouraud && !mr.ni_valid)
+                return fail(GR_ERR_INDEX_OUT_OF_RANGE,
+                            "normal index out of range under Gouraud shading");
+            o.vtx_block = vblocks;
+            vblocks += (mr.dm.V + 255) / 256;
+            o.nrm_block = vblocks;
+            if (o.gouraud) vblocks += (mr.dm.N + 255) / 256;
+            o.end_block = vblocks;
+            o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
+            o.face_block = fblocks;
+            o.seq_block = seq;
+            fblocks += o.face_blocks;
+            seq += o.face_blocks;
+            o.clip = (float4*)(uintptr_t)(clip_n * sizeof(float4));  // offsets, patched below
+            clip_n += mr.dm.V;
+            o.nint = (float*)(uintptr_t)(nint_n * sizeof(float));
+            if (o.gouraud) nint_n += mr.dm.N;
+            o.vis = (int32_t*)(uintptr_t)(dobj.size() * sizeof(int32_t));
+            dobj.push_back(o);
+        }
+        d.face_blocks = fblocks - d.face_block_begin;
+        d.rrec = (gr::RasterRec*)(uintptr_t)(slots * sizeof(gr::RasterRec));
+        d.srec = (gr::ShadeRec*)(uintptr_t)(slots * sizeof(gr::ShadeRec));
+        d.wrec = (gr::WideRec*)(uintptr_t)(slots * sizeof(gr::WideRec));
+        slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
+        if ((size_t)d.face_blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
+            return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
+    }
+    int nobj = (int)dobj.size();
+
+    // Texture descriptor table.
+    if (c->tex_dirty) {
+        std::vector<gr::DevTexture> tt(std::max<size_t>(1, c->textures.size()));
+        for (size_t i = 0; i < c->textures.size(); i++) tt[i] = c->textures[i].dt;
+        gr_status st = ensure(c, c->d_textures, tt.size() * sizeof(gr::DevTexture));
+        if (st) return st;
+        st = upload(c, c->d_textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture));
+        if (st) return st;
+        c->tex_dirty = false;
+    }
+
+    unsigned long long refs_cap = (unsigned long long)std::max<int64_t>(c->cfg.refs_capacity, 1 << 16);
+    gr_status st;
+    if ((st = ensure(c, c->objs, nobj * sizeof(gr::DevObject)))) return st;
+    if ((st = ensure(c, c->frames, nf * sizeof(gr::DevFrame)))) return st;
+    if ((st = ensure(c, c->clip, clip_n * sizeof(float4)))) return st;
+    if ((st = ensure(c, c->nint, nint_n * sizeof(float)))) return st;
+    if ((st = ensure(c, c->vis, nobj * sizeof(int32_t)))) return st;
+    if ((st = ensure(c, c->rrec, slots * sizeof(gr::RasterRec)))) return st;
+    if ((st = ensure(c, c->srec, slots * sizeof(gr::ShadeRec)))) return st;
+    if ((st = ensure(c, c->wrec, slots * sizeof(gr::WideRec)))) return st;
+    // tiles buffer: counts | overflow (zeroed) then offsets | cursor
+    size_t z32 = (size_t)tiles + nf;
+    if ((st = ensure(c, c->zero32, z32 * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, c->tiles, 2 * (size_t)tiles * sizeof(uint32_t)))) return st;
+    size_t z64 = (size_t)nf * gr::kNumStats + 1;
+    if ((st = ensure(c, c->zero64, z64 * sizeof(unsigned long long)))) return st;
+    if ((st = ensure(c, c->wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, c->refs, refs_cap * sizeof(uint32_t)))) return st;
+    if (!c->ctrl.p) {
+        if ((st = ensure(c, c->ctrl, sizeof(gr::DevControl)))) return st;
+        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
+    }
+
+    // Patch offsets into device pointers.
+    for (auto& o : dobj) {
+        o.clip = (float4*)((char*)c->clip.p + (uintptr_t)o.clip);
+        o.nint = (float*)((char*)c->nint.p + (uintptr_t)o.nint);
+        o.vis = (int32_t*)((char*)c->vis.p + (uintptr_t)o.vis);
+    }
+    for (auto& d : frames) {
+        d.rrec = (gr::RasterRec*)((char*)c->rrec.p + (uintptr_t)d.rrec);
+        d.srec = (gr::ShadeRec*)((char*)c->srec.p + (uintptr_t)d.srec);
+        d.wrec = (gr::WideRec*)((char*)c->wrec.p + (uintptr_t)d.wrec);
+    }
+    if ((st = upload(c, c->objs.p, dobj.data(), nobj * sizeof(gr::DevObject)))) return st;
+    if ((st = upload(c, c->frames.p, frames.data(), nf * sizeof(gr::DevFrame)))) return st;
+
+    gr::LaunchParams p{};
+    p.objs = (const gr::DevObject*)c->objs.p;
+    p.nobj = nobj;
+    p.frames = (const gr::DevFrame*)c->frames.p;
+    p.nframes = nf;
+    p.textures = (const gr::DevTexture*)c->d_textures.p;
+    p.vertex_blocks = vblocks;
+    p.face_blocks = fblocks;
+    p.total_tiles = tiles;
+    p.tw_log2 = twl;
+    p.th_log2 = thl;
+    p.tile_counts = (uint32_t*)c->zero32.p;
+    p.frame_overflow = (uint32_t*)c->zero32.p + tiles;
+    p.tile_offsets = (uint32_t*)c->tiles.p;
+    p.tile_cursor = (uint32_t*)c->tiles.p + tiles;
+    p.wg_count = (uint32_t*)c->wgcount.p;
+    p.refs = (uint32_t*)c->refs.p;
+    p.refs_capacity = refs_cap;
+    p.stats = (unsigned long long*)c->zero64.p;
+    p.refs_used = (unsigned long long*)c->zero64.p + (size_t)nf * gr::kNumStats;
+    p.ctrl = (gr::DevControl*)c->ctrl.p;
+    p.zero_u32 = (uint32_t*)c->zero32.p;
+    p.zero_u32_n = (int)z32;
+    p.zero_u64 = (unsigned long long*)c->zero64.p;
+    p.zero_u64_n = (int)z64;
+
+    timed_launch(c, "k_vertex", [&] { gr::launch_vertex(p, c->stream); });
+    timed_launch(c, "k_face", [&] { gr::launch_face(p, c->stream); });
+    timed_launch(c, "k_bin_scan", [&] { gr::launch_bin_scan(p, c->stream); });
+    timed_launch(c, "k_bin_fill", [&] { gr::launch_bin_fill(p, c->stream); });
+    bool ok = true;
+    timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
+    if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
+    GR_HIP(hipGetLastError());
+
+    // Stats of this batch to pinned host memory.
+    size_t sbytes = (size_t)nf * gr::kNumStats * sizeof(unsigned long long);
+    if (c->h_stats_cap < sbytes) {
+        GR_HIP(hipStreamSynchronize(c->stream));
+        if (c->h_stats) GR_HIP(hipHostFree(c->h_stats));
+        c->h_stats = nullptr;
+        GR_HIP(hipHostMalloc(&c->h_stats, sbytes, hipHostMallocDefault));
+        c->h_stats_cap = sbytes;
+    }
+    GR_HIP(hipMemcpyAsync(c->h_stats, p.stats, sbytes, hipMemcpyDeviceToHost, c->stream));
+    c->last_frames = nf;
+    return GR_OK;
+}
+
+// Wait, read and clear the sticky overflow words.
+gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
+    GR_HIP(hipMemcpyAsync(c->h_ctrl, c->ctrl.p, sizeof(gr::DevControl), hipMemcpyDeviceToHost,
+                          c->stream));
+    GR_HIP(hipStreamSynchronize(c->stream));
+    if (c->timing) collect_times(c);
+    gr::DevControl ctl;
+    std::memcpy(&ctl, c->h_ctrl, sizeof ctl);
+    *needed = 0;
+    if (ctl.overflow) {
+        *needed = ctl.refs_needed;
+        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
+        GR_HIP(hipStreamSynchronize(c->stream));
+        return GR_ERR_CAPACITY;
+    }
+    return GR_OK;
+}
+
+void grow_refs(gr_context* c, unsigned long long needed) {
+    int64_t want = (int64_t)needed + (int64_t)needed / 4 + 1024;
+    if (want > c->cfg.refs_capacity) c->cfg.refs_capacity = want;
+}
+
+gr_status upload_array(gr_context* c, MeshRes& mr, const void* src, size_t bytes, void** out) {
+    void* d = nullptr;
+    GR_HIP(hipMalloc(&d, std::max<size_t>(bytes, 4)));
+    mr.allocs.push_back(d);
+    mr.bytes += (int64_t)bytes;
+    GR_HIP(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice));
+    *out = d;
+    return GR_OK;
+}
+
+void free_mesh(MeshRes& mr) {
+    for (void* p : mr.allocs) (void)hipFree(p);
+    mr.allocs.clear();
+    mr.live = false;
+}
+
+bool is_pow2(int32_t x) { return x != 0 && (x & (x - 1)) == 0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* gr_last_error(void) { return g_last_error.c_str(); }
+int32_t gr_abi_version(void) { return GR_ABI_VERSION; }
+
+gr_status gr_create(int32_t device, gr_context** out) {
+    try {
+        if (!out) return fail(GR_ERR_INVALID_ARGUMENT, "null out");
+        *out = nullptr;
+        int n = 0;
+        GR_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) return fail(GR_ERR_INVALID_ARGUMENT, "no such device");
+        GR_HIP(hipSetDevice(device));
+        gr_context* c = new gr_context();
+        c->device = device;
+        if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return fail(GR_ERR_DEVICE, "hipStreamCreate failed");
+        }
+        c->stream = c->own;
+        if (hipHostMalloc(&c->h_ctrl, sizeof(gr::DevControl), hipHostMallocDefault) != hipSuccess) {
+            (void)hipStreamDestroy(c->own);
+            delete c;
+            return fail(GR_ERR_OUT_OF_MEMORY, "pinned allocation failed");
+        }
+        *out = c;
+        return GR_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (...) {
+        return fail(GR_ERR_DEVICE, "unexpected exception");
+    }
+}
+
+void gr_destroy(gr_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& m : c->meshes) free_mesh(m);
+    for (auto& t : c->textures)
+        if (t.texels) (void)hipFree(t.texels);
+    DevBuf* bufs[] = {&c->d_textures, &c->objs, &c->frames, &c->clip, &c->nint, &c->vis,
+                      &c->rrec, &c->srec, &c->wrec, &c->tiles, &c->zero32, &c->zero64,
+                      &c->wgcount, &c->refs, &c->out_rgba, &c->out_depth, &c->ctrl};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto& s : c->ring) {
+        if (s.host) (void)hipHostFree(s.host);
+        if (s.ev) (void)hipEventDestroy(s.ev);
+    }
+    for (auto& t : c->timed) {
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
+    if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+}
+
+gr_status gr_set_config(gr_context* c, const gr_config* cfg) {
+    if (!c || !cfg) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    gr_config n = c->cfg;
+    if (cfg->tile_w) n.tile_w = cfg->tile_w;
+    if (cfg->tile_h) n.tile_h = cfg->tile_h;
+    if (cfg->refs_capacity) n.refs_capacity = cfg->refs_capacity;
+    if (cfg->raster_waves) n.raster_waves = cfg->raster_waves;
+    if (!valid_tile(n.tile_w) || !valid_tile(n.tile_h))
+        return fail(GR_ERR_INVALID_ARGUMENT, "tile sides must be 8, 16, 32 or 64");
+    if (n.raster_waves != 4) return fail(GR_ERR_INVALID_ARGUMENT, "raster_waves must be 4");
+    if (n.refs_capacity < 1024) return fail(GR_ERR_INVALID_ARGUMENT, "refs_capacity too small");
+    c->cfg = n;
+    return GR_OK;
+}
+
+gr_status gr_get_config(gr_context* c, gr_config* cfg) {
+    if (!c || !cfg) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    *cfg = c->cfg;
+    return GR_OK;
+}
+
+gr_status gr_set_stream(gr_context* c, void* s) {
+    if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    c->stream = s ? (hipStream_t)s : c->own;
+    return GR_OK;
+}
+
+void* gr_get_stream(gr_context* c) { return c ? (void*)c->stream : nullptr; }
+
+int32_t gr_texture_kind_for_size(int32_t w, int32_t h) {
+    return (is_pow2(w) && is_pow2(h)) ? GR_TEXTURE_IMAGE_FAST : GR_TEXTURE_IMAGE;
+}
+
+gr_status gr_texture_create(gr_context* c, const gr_texture_desc* d, const uint8_t* rgba,
+                            int32_t* out) {
+    try {
+        if (!c || !d || !out) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+        if (d->type != GR_TEXTURE_SOLID && d->type != GR_TEXTURE_IMAGE &&
+            d->type != GR_TEXTURE_IMAGE_FAST)
+            return fail(GR_ERR_INVALID_ARGUMENT, "unknown texture type");
+        GR_HIP(hipSetDevice(c->device));
+        TexRes t;
+        t.live = true;
+        t.dt.type = d->type;
+        t.dt.w = d->width;
+        t.dt.h = d->height;
+        t.dt.wf = (float)d->width;
+        t.dt.hf = (float)d->height;
+        t.dt.scale = d->scale;
+        t.dt.color = pack_rgba(d->color);
+        t.dt.texels = nullptr;
+        if (d->type == GR_TEXTURE_IMAGE || d->type == GR_TEXTURE_IMAGE_FAST) {
+            if (d->width < 1 || d->height < 1 || !rgba)
+                return fail(GR_ERR_INVALID_ARGUMENT, "image texture needs size and texels");
+            if (d->type == GR_TEXTURE_IMAGE_FAST && !(is_pow2(d->width) && is_pow2(d->height)))
+                return fail(GR_ERR_INVALID_ARGUMENT, "IMAGE_FAST needs power-of-two sizes");
+            size_t bytes = (size_t)d->width * d->height * 4;
+            GR_HIP(hipMalloc(&t.texels, bytes));
+            hipError_t e = hipMemcpy(t.texels, rgba, bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                (void)hipFree(t.texels);
+                return fail(GR_ERR_DEVICE, hipGetErrorString(e));
+            }
+            t.dt.texels = (const uint32_t*)t.texels;
+            t.bytes = (int64_t)bytes;
+        }
+        c->textures.push_back(t);
+        c->tex_dirty = true;
+        *out = (int32_t)c->textures.size() - 1;
+        return GR_OK;
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_texture_destroy(gr_context* c, int32_t h) {
+    if (!c || h < 0 || h >= (int32_t)c->textures.size() || !c->textures[h].live)
+        return fail(GR_ERR_INVALID_HANDLE, "invalid texture handle");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    if (c->textures[h].texels) GR_HIP(hipFree(c->textures[h].texels));
+    c->textures[h] = TexRes{};
+    c->tex_dirty = true;
+    return GR_OK;
+}
+
+gr_status gr_mesh_create(gr_context* c, const gr_mesh_desc* d, int32_t* out) {
+    try {
+        if (!c || !d || !out) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+        GR_HIP(hipSetDevice(c->device));
+        const int32_t V = d->num_vertices, N = d->num_vertex_normals, F = d->num_faces;
+        if (V < 1 || !d->vertices)
+            return fail(GR_ERR_INVALID_ARGUMENT, "mesh needs at least one vertex");
+        if (N < 0 || (N > 0 && !d->vertex_normals) || F < 0 || (F > 0 && !d->vertex_indices))
+            return fail(GR_ERR_INVALID_ARGUMENT, "inconsistent mesh arrays");
+        for (int64_t k = 0; k < 3 * (int64_t)F; k++)
+            if (d->vertex_indices[k] < 0 || d->vertex_indices[k] >= V)
+                return fail(GR_ERR_INDEX_OUT_OF_RANGE, "vertex index out of range");
+
+        MeshRes mr;
+        mr.live = true;
+        gr::DevMesh& m = mr.dm;
+        std::memset(&m, 0, sizeof m);
+        m.V = V;
+        m.N = N;
+        m.F = F;
+        gr_status st;
+        // SoA positions; W kept only when some W != 1.
+        {
+            std::vector<float> x(V), y(V), z(V), w(V);
+            bool w1 = true;
+            for (int32_t i = 0; i < V; i++) {
+                x[i] = d->vertices[4 * i];
+                y[i] = d->vertices[4 * i + 1];
+                z[i] = d->vertices[4 * i + 2];
+                w[i] = d->vertices[4 * i + 3];
+                uint32_t wb;
+                std::memcpy(&wb, &w[i], 4);
+                w1 = w1 && wb == 0x3F800000u;
+            }
+            void* p;
+            if ((st = upload_array(c, mr, x.data(), 4 * (size_t)V, &p))) goto failed;
+            m.px = (const float*)p;
+            if ((st = upload_array(c, mr, y.data(), 4 * (size_t)V, &p))) goto failed;
+            m.py = (const float*)p;
+            if ((st = upload_array(c, mr, z.data(), 4 * (size_t)V, &p))) goto failed;
+            m.pz = (const float*)p;
+            if (!w1) {
+                if ((st = upload_array(c, mr, w.data(), 4 * (size_t)V, &p))) goto failed;
+                m.pw = (const float*)p;
+            }
+            // boundingBox (mesh.go:28-51)
+            float mn[3] = {x[0], y[0], z[0]}, mx[3] = {x[0], y[0], z[0]};
+            for (int32_t i = 0; i < V; i++) {
+                const float q[3] = {x[i], y[i], z[i]};
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = go_min(mn[k], q[k]);
+                    mx[k] = go_max(mx[k], q[k]);
+                }
+            }
+            int k = 0;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    for (int cc = 0; cc < 2; cc++) {
+                        m.bbox[k][0] = a ? mx[0] : mn[0];
+                        m.bbox[k][1] = b ? mx[1] : mn[1];
+                        m.bbox[k][2] = cc ? mx[2] : mn[2];
+                        m.bbox[k][3] = 1.0f;
+                        k++;
+                    }
+        }
+        if (N > 0) {
+            std::vector<float> x(N), y(N), z(N), w(N);
+            bool w1 = true;
+            for (int32_t i = 0; i < N; i++) {
+                x[i] = d->vertex_normals[4 * i];
+                y[i] = d->vertex_normals[4 * i + 1];
+                z[i] = d->vertex_normals[4 * i + 2];
+                w[i] = d->vertex_normals[4 * i + 3];
+                uint32_t wb;
+                std::memcpy(&wb, &w[i], 4);
+                w1 = w1 && wb == 0x3F800000u;
+            }
+            void* p;
+            if ((st = upload_array(c, mr, x.data(), 4 * (size_t)N, &p))) goto failed;
+            m.nx = (const float*)p;
+            if ((st = upload_array(c, mr, y.data(), 4 * (size_t)N, &p))) goto failed;
+            m.ny = (const float*)p;
+            if ((st = upload_array(c, mr, z.data(), 4 * (size_t)N, &p))) goto failed;
+            m.nz = (const float*)p;
+            if (!w1) {
+                if ((st = upload_array(c, mr, w.data(), 4 * (size_t)N, &p))) goto failed;
+                m.nw = (const float*)p;
+            }
+        }
+        if (F > 0) {
+            void* p;
+            if ((st = upload_array(c, mr, d->vertex_indices, 12 * (size_t)F, &p))) goto failed;
+            m.vi = (const int32_t*)p;
+            // Normal indices: a Go zero Face has all zeros when none are given.
+            std::vector<int32_t> ni(3 * (size_t)F, 0);
+            if (d->normal_indices) std::memcpy(ni.data(), d->normal_indices, ni.size() * 4);
+            mr.ni_valid = true;
+            if (N > 0)
+                for (int32_t v : ni) mr.ni_valid = mr.ni_valid && v >= 0 && v < N;
+            bool same = std::memcmp(ni.data(), d->vertex_indices, ni.size() * 4) == 0;
+            if (!same) {
+                if ((st = upload_array(c, mr, ni.data(), ni.size() * 4, &p))) goto failed;
+                m.ni = (const int32_t*)p;
+            }
+            // UVs: a per-vertex table when every corner agrees with one.
+            if (d->uvs) {
+                std::vector<float> tab(2 * (size_t)V, 0.f);
+                std::vector<uint8_t> seen(V, 0);
+                bool per_vertex = true;
+                for (int64_t k = 0; k < 3 * (int64_t)F && per_vertex; k++) {
+                    int32_t v = d->vertex_indices[k];
+                    uint32_t nu, nv, ou, ov;
+                    std::memcpy(&nu, &d->uvs[2 * k], 4);
+                    std::memcpy(&nv, &d->uvs[2 * k + 1], 4);
+                    if (!seen[v]) {
+                        seen[v] = 1;
+                        tab[2 * v] = d->uvs[2 * k];
+                        tab[2 * v + 1] = d->uvs[2 * k + 1];
+                    } else {
+                        std::memcpy(&ou, &tab[2 * v], 4);
+                        std::memcpy(&ov, &tab[2 * v + 1], 4);
+                        per_vertex = nu == ou && nv == ov;
+                    }
+                }
+                if (per_vertex) {
+                    if ((st = upload_array(c, mr, tab.data(), tab.size() * 4, &p))) goto failed;
+                    m.uv_vert = (const float*)p;
+                } else {
+                    if ((st = upload_array(c, mr, d->uvs, 24 * (size_t)F, &p))) goto failed;
+                    m.uv_face = (const float*)p;
+                }
+            }
+            // Texture per face, or one texture for the whole mesh.
+            m.mesh_tex = -1;
+            if (d->face_textures) {
+                bool uniform = true;
+                for (int32_t f = 0; f < F; f++) {
+                    int32_t t = d->face_textures[f];
+                    if (t < -1 || t >= (int32_t)c->textures.size() || (t >= 0 && !c->textures[t].live)) {
+                        st = fail(GR_ERR_INVALID_HANDLE, "invalid texture handle in face_textures");
+                        goto failed;
+                    }
+                    uniform = uniform && t == d->face_textures[0];
+                    if (t >= 0 && std::find(mr.textures.begin(), mr.textures.end(), t) == mr.textures.end())
+                        mr.textures.push_back(t);
+                }
+                if (uniform) {
+                    m.mesh_tex = d->face_textures[0];
+                } else {
+                    if ((st = upload_array(c, mr, d->face_textures, 4 * (size_t)F, &p))) goto failed;
+                    m.face_tex = (const int32_t*)p;
+                }
+            }
+            // Face normals (mesh.go:53-60) as given, or computed on the GPU.
+            if (d->face_normals) {
+                if ((st = upload_array(c, mr, d->face_normals, 16 * (size_t)F, &p))) goto failed;
+                m.fn = (const float*)p;
+            } else {
+                void* fnp = nullptr;
+                if (hipMalloc(&fnp, 16 * (size_t)F) != hipSuccess) {
+                    st = fail(GR_ERR_OUT_OF_MEMORY, "hipMalloc face normals");
+                    goto failed;
+                }
+                mr.allocs.push_back(fnp);
+                mr.bytes += 16 * (int64_t)F;
+                gr::launch_face_normals(m.px, m.py, m.pz, m.vi, F, (float*)fnp, c->stream);
+                if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
+                    st = fail(GR_ERR_DEVICE, "face normal kernel failed");
+                    goto failed;
+                }
+                m.fn = (const float*)fnp;
+            }
+        }
+        c->meshes.push_back(mr);
+        *out = (int32_t)c->meshes.size() - 1;
+        return GR_OK;
+    failed:
+        free_mesh(mr);
+        return st;
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_mesh_destroy(gr_context* c, int32_t h) {
+    if (!c || h < 0 || h >= (int32_t)c->meshes.size() || !c->meshes[h].live)
+        return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    free_mesh(c->meshes[h]);
+    c->meshes[h] = MeshRes{};
+    return GR_OK;
+}
+
+int64_t gr_mesh_device_bytes(gr_context* c, int32_t h) {
+    if (!c || h < 0 || h >= (int32_t)c->meshes.size() || !c->meshes[h].live) return -1;
+    return c->meshes[h].bytes;
+}
+
+void gr_frame_defaults(gr_frame* f, int32_t w, int32_t h) {
+    if (!f) return;
+    std::memset(f, 0, sizeof *f);
+    f->width = w;
+    f->height = h;
+    f->flags = GR_DEFAULT_FLAGS;
+    f->ref_tiles = 16;
+    // NewScreenMatrix (matrix.go:108-118)
+    float hw = (float)w / 2.0f, hh = (float)h / 2.0f;
+    f->screen[0] = hw;
+    f->screen[3] = hw;
+    f->screen[5] = hh;
+    f->screen[7] = hh;
+    f->screen[10] = 0.5f;
+    f->screen[11] = 0.5f;
+    f->screen[15] = 1.0f;
+    // Vec3{-1,1,1}.Normalize() (renderer.go:265, vector.go:92-94)
+    float len = (float)std::sqrt((double)(1.0f + 1.0f + 1.0f));
+    f->light[0] = -1.0f / len;
+    f->light[1] = 1.0f / len;
+    f->light[2] = 1.0f / len;
+    f->ambient = 0.5f;
+    f->diffuse = 0.5f;
+    f->znear = 0.0f;
+    f->zfar = 50.0f;
+    const uint8_t face[4] = {200, 200, 200, 255}, clr[4] = {50, 50, 50, 255},
+                  dot[4] = {100, 100, 100, 255};
+    std::memcpy(f->face_color, face, 4);
+    std::memcpy(f->clear_color, clr, 4);
+    std::memcpy(f->dot_color, dot, 4);
+    f->dot_step = 10;
+}
+
+gr_status gr_draw_batch(gr_context* c, const gr_batch_frame* frames, int32_t nf,
+                        const gr_object* objs, int32_t no) {
+    try {
+        if (!c || !frames || (no > 0 && !objs)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+        GR_HIP(hipSetDevice(c->device));
+        return enqueue(c, frames, nf, objs, no);
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_draw_device(gr_context* c, const gr_frame* frame, const gr_object* objs,
+                         int32_t no, void* d_rgba, void* d_depth) {
+    if (!frame) return fail(GR_ERR_INVALID_ARGUMENT, "null frame");
+    gr_batch_frame bf;
+    bf.frame = *frame;
+    bf.first_object = 0;
+    bf.num_objects = no;
+    bf.d_rgba = d_rgba;
+    bf.d_depth = d_depth;
+    return gr_draw_batch(c, &bf, 1, objs, no);
+}
+
+gr_status gr_synchronize(gr_context* c) {
+    if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
+    if (!c->h_ctrl || !c->ctrl.p) {
+        GR_HIP(hipStreamSynchronize(c->stream));
+        return GR_OK;
+    }
+    unsigned long long needed = 0;
+    gr_status st = sync_and_check(c, &needed);
+    if (st == GR_ERR_CAPACITY) {
+        grow_refs(c, needed);
+        return fail(GR_ERR_CAPACITY, "tile-list capacity exceeded; redraw the batch");
+    }
+    return st;
+}
+
+gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, int32_t no,
+                  uint8_t* rgba, float* depth, gr_stats* stats) {
+    try {
+        if (!c || !frame || (no > 0 && !objs)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+        GR_HIP(hipSetDevice(c->device));
+        gr_status st = validate_frame(*frame);
+        if (st) return st;
+        size_t npx = (size_t)frame->width * frame->height;
+        if ((st = ensure(c, c->out_rgba, npx * 4))) return st;
+        if ((st = ensure(c, c->out_depth, npx * 4))) return st;
+        for (int attempt = 0; attempt < 4; attempt++) {
+            st = gr_draw_device(c, frame, objs, no, c->out_rgba.p, c->out_depth.p);
+            if (st) return st;
+            unsigned long long needed = 0;
+            st = sync_and_check(c, &needed);
+            if (st == GR_ERR_CAPACITY) {
+                grow_refs(c, needed);
+                continue;
+            }
+            if (st) return st;
+            if (rgba) GR_HIP(hipMemcpy(rgba, c->out_rgba.p, npx * 4, hipMemcpyDeviceToHost));
+            if (depth) GR_HIP(hipMemcpy(depth, c->out_depth.p, npx * 4, hipMemcpyDeviceToHost));
+            if (stats) {
+                const unsigned long long* s = (const unsigned long long*)c->h_stats;
+                stats->tpf = (int64_t)s[gr::kStatTpf];
+                stats->triangles = (int64_t)s[gr::kStatTriangles];
+                stats->fragments_covered = (int64_t)s[gr::kStatCovered];
+                stats->pixels_shaded = (int64_t)s[gr::kStatShaded];
+                stats->faces_front = (int64_t)s[gr::kStatFront];
+                stats->clip_overflow = (int64_t)s[gr::kStatClipOverflow];
+                stats->refs = (int64_t)s[gr::kStatRefs];
+                stats->reserved = 0;
+            }
+            return GR_OK;
+        }
+        return fail(GR_ERR_CAPACITY, "tile-list capacity could not be satisfied");
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_last_stats(gr_context* c, gr_stats* stats, int32_t max_frames, int32_t* nf) {
+    if (!c || !stats) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    int32_t n = std::min(max_frames, c->last_frames);
+    const unsigned long long* s = (const unsigned long long*)c->h_stats;
+    for (int32_t f = 0; f < n && s; f++) {
+        const unsigned long long* q = s + (size_t)f * gr::kNumStats;
+        stats[f].tpf = (int64_t)q[gr::kStatTpf];
+        stats[f].triangles = (int64_t)q[gr::kStatTriangles];
+        stats[f].fragments_covered = (int64_t)q[gr::kStatCovered];
+        stats[f].pixels_shaded = (int64_t)q[gr::kStatShaded];
+        stats[f].faces_front = (int64_t)q[gr::kStatFront];
+        stats[f].clip_overflow = (int64_t)q[gr::kStatClipOverflow];
+        stats[f].refs = (int64_t)q[gr::kStatRefs];
+        stats[f].reserved = 0;
+    }
+    if (nf) *nf = s ? n : 0;
+    return GR_OK;
+}
+
+gr_status gr_set_kernel_timing(gr_context* c, int32_t enabled) {
+    if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    collect_times(c);
+    c->timing = enabled != 0;
+    return GR_OK;
+}
+
+int32_t gr_kernel_times(gr_context* c, const char** names, double* total_ms, int64_t* launches,
+                        int32_t max) {
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->stream);
+    collect_times(c);
+    int32_t i = 0;
+    static std::mutex mu;
+    static std::map<std::string, std::string> interned;
+    for (auto& kv : c->kernel_ms) {
+        if (i >= max) break;
+        std::lock_guard<std::mutex> g(mu);
+        auto it = interned.emplace(kv.first, kv.first).first;
+        if (names) names[i] = it->second.c_str();
+        if (total_ms) total_ms[i] = kv.second.first;
+        if (launches) launches[i] = kv.second.second;
+        i++;
+    }
+    return i;
+}
+
+void gr_reset_kernel_times(gr_context* c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->stream);
+    collect_times(c);
+    c->kernel_ms.clear();
+}
+
+}  // extern "C"

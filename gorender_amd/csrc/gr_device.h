@@ -1,0 +1,151 @@
+// Device-side data layout of the render path (shared by host launch code and
+// kernels). All structs are plain data copied to HBM per draw.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gr {
+
+// Faces handled by one geometry workgroup, and the output-triangle slots it
+// reserves: a face yields at most 7 fan triangles after clipping against 6
+// planes (clipping.go:8-10), so slot = wg * kSlotsPerWg + local prefix keeps the
+// reference's (object, face, fan) draw order as a plain integer order.
+constexpr int kFacesPerWg = 256;
+constexpr int kMaxFan = 7;
+constexpr int kSlotsPerWg = kFacesPerWg * kMaxFan;  // 1792
+constexpr int kMaxClip = 9;                         // maxClipPoints (clipping.go:10)
+constexpr int kMaxDim = 16384;                      // framebuffer side limit
+
+// Coordinates within +-kFastCoord keep every edge value of a pixel inside a
+// 16384^2 framebuffer below 2^31, so int32 arithmetic equals the reference's
+// int64 (rasterizer.go:107-117) exactly.
+constexpr int kFastCoord = 16384;
+
+// Box visibility (clipping.go:22-26).
+enum : int32_t { kBoxOutside = 0, kBoxIntersect = 1, kBoxInside = 2 };
+
+// Flags mirror include/gorender_hip.h.
+enum : uint32_t {
+    kShowFaces = 1u << 0,
+    kBackfaceCulling = 1u << 1,
+    kLighting = 1u << 2,
+    kFlatShading = 1u << 3,
+    kFrustumClipping = 1u << 4,
+    kShowTextures = 1u << 5,
+};
+
+struct DevMesh {
+    int32_t V, N, F;
+    int32_t mesh_tex;        // texture used by every face when face_tex == nullptr
+    const float* px;         // SoA positions
+    const float* py;
+    const float* pz;
+    const float* pw;         // nullptr when every W == 1
+    const float* nx;         // SoA vertex normals (N entries)
+    const float* ny;
+    const float* nz;
+    const float* nw;         // nullptr when every W == 1
+    const int32_t* vi;       // [F*3]
+    const int32_t* ni;       // [F*3]; nullptr when identical to vi
+    const float* uv_vert;    // [V*2] per-vertex UVs, or nullptr
+    const float* uv_face;    // [F*6] per-corner UVs, or nullptr
+    const int32_t* face_tex; // [F] texture slot per face or nullptr
+    const float* fn;         // [F*4] face normals
+    float bbox[8][4];        // boundingBox corners (mesh.go:28-51)
+};
+
+struct DevObject {
+    DevMesh m;
+    float mvp[16];
+    float world[16];
+    int32_t frame;          // index in the batch
+    int32_t gouraud;        // vertex-normal lighting in use
+    int32_t vtx_block;      // first k_vertex block of this object
+    int32_t nrm_block;      // first normal block (after the vertex blocks)
+    int32_t end_block;      // one past the last k_vertex block
+    int32_t face_block;     // first k_face block (batch-global)
+    int32_t face_blocks;    // number of k_face blocks
+    int32_t seq_block;      // first face block index within the frame (draw order)
+    float4* clip;           // scratch [V] clip-space positions
+    float* nint;            // scratch [N] Gouraud intensity per normal
+    int32_t* vis;           // scratch: box visibility
+    int32_t pad;
+};
+
+struct DevTexture {
+    int32_t type, w, h, pad;
+    float wf, hf, scale;
+    uint32_t color;
+    const uint32_t* texels;
+};
+
+struct DevFrame {
+    int32_t W, H;
+    uint32_t flags;
+    int32_t ref_tiles;
+    float screen[16];
+    float light[3];
+    float ambient, diffuse, znear, zfar;
+    uint32_t face_color, clear_color, dot_color;
+    int32_t dot_step;
+    float ref_tile_rect[16][4];  // reference TPF grid (renderer.go:50-76)
+    int32_t tiles_x, tiles_y;    // raster tiles of this frame
+    int32_t tile_begin;          // first tile in the batch tile arrays
+    int32_t face_block_begin;    // first k_face block of this frame
+    int32_t face_blocks;         // k_face blocks of this frame
+    int32_t pad0;
+    uint8_t* rgba;               // device outputs
+    float* depth;
+    struct RasterRec* rrec;      // frame-local slot arrays
+    struct ShadeRec* srec;
+    struct WideRec* wrec;
+};
+
+// Per-triangle raster set-up, 64 bytes: what the visibility pass reads.
+struct alignas(16) RasterRec {
+    int16_t bx0, bx1, by0, by1;  // clamped pixel bounding box; bx0 > bx1 = empty
+    uint32_t wide;               // 1: coordinates beyond kFastCoord, use WideRec
+    float sf;                    // float32(fx12+fx20+fx01), constant per triangle
+    float z0, z1, z2;            // clip-space W of the vertices (renderer.go:369)
+    int32_t e[9];                // A01,B01,C01', A12,B12,C12', A20,B20,C20'
+};
+
+// Per-triangle shading inputs, 48 bytes.
+struct alignas(16) ShadeRec {
+    float uz[3];  // u_i / z_i (rasterizer.go:132-137)
+    float vz[3];
+    float in[3];  // intensities
+    int32_t tex;  // texture slot or -1
+    int32_t pad[2];
+};
+
+// Exact int64 coordinates for triangles outside the int32 fast range.
+struct alignas(16) WideRec {
+    int64_t x[3], y[3];
+    uint32_t adj;  // top-left adjustment bits of edges 01, 12, 20
+    uint32_t pad[3];
+};
+
+// Per-frame counters (device), in order of gr_stats.
+enum : int32_t {
+    kStatTpf = 0,
+    kStatTriangles,
+    kStatCovered,
+    kStatShaded,
+    kStatFront,
+    kStatClipOverflow,
+    kStatRefs,
+    kStatReserved,
+    kNumStats
+};
+
+// Sticky control words (cleared by the host when it reads them): set when a
+// frame of any batch needed more tile-list entries than were allocated.
+struct DevControl {
+    unsigned long long refs_needed;
+    uint32_t overflow;
+    uint32_t pad;
+};
+
+}  // namespace gr

@@ -1,0 +1,46 @@
+// Launch parameters shared by the host orchestration and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gr_device.h"
+
+namespace gr {
+
+struct LaunchParams {
+    const DevObject* objs;
+    int nobj;
+    const DevFrame* frames;
+    int nframes;
+    const DevTexture* textures;
+    int vertex_blocks;       // k_vertex blocks covering all objects
+    int face_blocks;         // k_face / k_bin_fill blocks
+    int total_tiles;         // raster workgroups
+    int tw_log2, th_log2;    // raster tile size
+    uint32_t* tile_counts;   // [total_tiles]
+    uint32_t* tile_offsets;  // [total_tiles]
+    uint32_t* tile_cursor;   // [total_tiles]
+    uint32_t* frame_overflow;// [nframes]
+    uint32_t* wg_count;      // [face_blocks]
+    uint32_t* refs;          // [refs_capacity]
+    unsigned long long refs_capacity;
+    unsigned long long* stats;  // [nframes * kNumStats]
+    unsigned long long* refs_used;  // batch allocation cursor of refs
+    DevControl* ctrl;
+    // zeroed by k_vertex before anything else runs
+    uint32_t* zero_u32;
+    int zero_u32_n;
+    unsigned long long* zero_u64;
+    int zero_u64_n;
+};
+
+void launch_face_normals(const float* px, const float* py, const float* pz, const int32_t* vi,
+                         int F, float* fn, hipStream_t s);
+void launch_vertex(const LaunchParams& p, hipStream_t s);
+void launch_face(const LaunchParams& p, hipStream_t s);
+void launch_bin_scan(const LaunchParams& p, hipStream_t s);
+void launch_bin_fill(const LaunchParams& p, hipStream_t s);
+bool launch_raster(const LaunchParams& p, hipStream_t s);
+
+}  // namespace gr

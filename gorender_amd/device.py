@@ -1,0 +1,203 @@
+"""Python handle on a gorender_hip context (one GPU, one HIP stream).
+
+Thin layer over the C ABI: uploads meshes/textures to HBM once, then draws
+scenes synchronously into numpy buffers or asynchronously into device memory.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .scenes import FrameData, MeshData, SceneData, TextureData
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def frame_struct(fr: FrameData) -> _lib.gr_frame:
+    f = _lib.gr_frame()
+    f.width, f.height, f.flags, f.ref_tiles = fr.width, fr.height, fr.flags, fr.ref_tiles
+    for i, v in enumerate(np.asarray(fr.screen, np.float32).reshape(16)):
+        f.screen[i] = float(v)
+    for i, v in enumerate(np.asarray(fr.light, np.float32).reshape(3)):
+        f.light[i] = float(v)
+    f.ambient, f.diffuse = float(np.float32(fr.ambient)), float(np.float32(fr.diffuse))
+    f.znear, f.zfar = float(np.float32(fr.znear)), float(np.float32(fr.zfar))
+    for i in range(4):
+        f.face_color[i] = fr.face_color[i]
+        f.clear_color[i] = fr.clear_color[i]
+        f.dot_color[i] = fr.dot_color[i]
+    f.dot_step = fr.dot_step
+    return f
+
+
+class Context:
+    """A gorender_hip context on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.gr_create(device, ctypes.byref(h)))
+        self.handle = h
+        self._mesh_cache: Dict[int, int] = {}
+        self._tex_cache: Dict[int, int] = {}
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.gr_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- configuration -------------------------------------------------
+    def set_config(self, tile_w: int = 0, tile_h: int = 0, refs_capacity: int = 0) -> None:
+        c = _lib.gr_config(tile_w, tile_h, refs_capacity, 0, 0)
+        _lib.check(self.lib.gr_set_config(self.handle, ctypes.byref(c)))
+
+    def config(self) -> _lib.gr_config:
+        c = _lib.gr_config()
+        _lib.check(self.lib.gr_get_config(self.handle, ctypes.byref(c)))
+        return c
+
+    def set_stream(self, stream_ptr: Optional[int]) -> None:
+        _lib.check(self.lib.gr_set_stream(self.handle, stream_ptr))
+
+    def stream(self) -> int:
+        return self.lib.gr_get_stream(self.handle) or 0
+
+    # ---- assets ----------------------------------------------------------
+    def upload_texture(self, t: TextureData) -> int:
+        d = _lib.gr_texture_desc(t.type, t.width, t.height, float(np.float32(t.scale)))
+        for i in range(4):
+            d.color[i] = t.color[i]
+        rgba = None
+        if t.rgba is not None:
+            rgba = np.ascontiguousarray(t.rgba, np.uint8)
+        h = ctypes.c_int32()
+        _lib.check(self.lib.gr_texture_create(self.handle, ctypes.byref(d), _ptr(rgba),
+                                              ctypes.byref(h)))
+        return h.value
+
+    def upload_mesh(self, m: MeshData, tex_handles: Sequence[int] = ()) -> int:
+        v = np.ascontiguousarray(m.vertices, np.float32)
+        vi = np.ascontiguousarray(m.vertex_indices, np.int32)
+        vn = None if m.vertex_normals is None else np.ascontiguousarray(m.vertex_normals,
+                                                                        np.float32)
+        ni = None if m.normal_indices is None else np.ascontiguousarray(m.normal_indices,
+                                                                        np.int32)
+        uv = None if m.uvs is None else np.ascontiguousarray(m.uvs, np.float32)
+        ft = None
+        if m.face_textures is not None:
+            lut = np.asarray(list(tex_handles) + [-1], np.int32)
+            ft = np.ascontiguousarray(np.where(m.face_textures >= 0,
+                                               lut[np.clip(m.face_textures, 0, None)], -1),
+                                      np.int32)
+        fn = None if m.face_normals is None else np.ascontiguousarray(m.face_normals, np.float32)
+        d = _lib.gr_mesh_desc(len(v), _ptr(v), 0 if vn is None else len(vn), _ptr(vn), len(vi),
+                              _ptr(vi), _ptr(ni), _ptr(uv), _ptr(ft), _ptr(fn))
+        h = ctypes.c_int32()
+        _lib.check(self.lib.gr_mesh_create(self.handle, ctypes.byref(d), ctypes.byref(h)))
+        return h.value
+
+    def destroy_mesh(self, h: int) -> None:
+        _lib.check(self.lib.gr_mesh_destroy(self.handle, h))
+
+    def destroy_texture(self, h: int) -> None:
+        _lib.check(self.lib.gr_texture_destroy(self.handle, h))
+
+    def mesh_bytes(self, h: int) -> int:
+        return int(self.lib.gr_mesh_device_bytes(self.handle, h))
+
+    def upload_scene(self, s: SceneData) -> Tuple[List[int], List[int]]:
+        th = [self.upload_texture(t) for t in s.textures]
+        mh = [self.upload_mesh(m, th) for m in s.meshes]
+        return mh, th
+
+    # ---- drawing -----------------------------------------------------------
+    @staticmethod
+    def objects_struct(s: SceneData, mesh_handles: Sequence[int]):
+        arr = (_lib.gr_object * max(1, len(s.objects)))()
+        for i, o in enumerate(s.objects):
+            arr[i].mesh = mesh_handles[o.mesh]
+            for k, val in enumerate(np.asarray(o.mvp, np.float32).reshape(16)):
+                arr[i].mvp[k] = float(val)
+            for k, val in enumerate(np.asarray(o.world, np.float32).reshape(16)):
+                arr[i].world[k] = float(val)
+        return arr
+
+    def draw(self, s: SceneData, mesh_handles: Optional[Sequence[int]] = None):
+        """Synchronous Draw; returns (rgba (H,W,4) uint8, depth (H,W) float32, stats)."""
+        if mesh_handles is None:
+            mesh_handles, _ = self.upload_scene(s)
+        fr = frame_struct(s.frame)
+        objs = self.objects_struct(s, mesh_handles)
+        rgba = np.empty((s.frame.height, s.frame.width, 4), np.uint8)
+        depth = np.empty((s.frame.height, s.frame.width), np.float32)
+        st = _lib.gr_stats()
+        _lib.check(self.lib.gr_draw(self.handle, ctypes.byref(fr), objs, len(s.objects),
+                                    _ptr(rgba), _ptr(depth), ctypes.byref(st)))
+        return rgba, depth, st.as_dict()
+
+    def draw_device(self, s: SceneData, mesh_handles: Sequence[int], d_rgba: int,
+                    d_depth: int) -> None:
+        fr = frame_struct(s.frame)
+        objs = self.objects_struct(s, mesh_handles)
+        _lib.check(self.lib.gr_draw_device(self.handle, ctypes.byref(fr), objs,
+                                           len(s.objects), d_rgba, d_depth))
+
+    def draw_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
+                   outputs: Sequence[Tuple[int, int]]) -> None:
+        """One asynchronous batch: frame k draws scenes[k] into outputs[k]."""
+        nf = len(scenes)
+        frames = (_lib.gr_batch_frame * nf)()
+        total = sum(len(s.objects) for s in scenes)
+        objs = (_lib.gr_object * max(1, total))()
+        k = 0
+        for f, s in enumerate(scenes):
+            frames[f].frame = frame_struct(s.frame)
+            frames[f].first_object = k
+            frames[f].num_objects = len(s.objects)
+            frames[f].d_rgba, frames[f].d_depth = outputs[f]
+            part = self.objects_struct(s, mesh_handles[f])
+            for i in range(len(s.objects)):
+                objs[k + i] = part[i]
+            k += len(s.objects)
+        _lib.check(self.lib.gr_draw_batch(self.handle, frames, nf, objs, total))
+
+    def synchronize(self) -> None:
+        _lib.check(self.lib.gr_synchronize(self.handle))
+
+    def last_stats(self, max_frames: int = 1024) -> List[dict]:
+        arr = (_lib.gr_stats * max_frames)()
+        n = ctypes.c_int32()
+        _lib.check(self.lib.gr_last_stats(self.handle, arr, max_frames, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    # ---- measurement -------------------------------------------------------
+    def set_kernel_timing(self, on: bool) -> None:
+        _lib.check(self.lib.gr_set_kernel_timing(self.handle, 1 if on else 0))
+
+    def kernel_times(self) -> Dict[str, Tuple[float, int]]:
+        n = 32
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int64 * n)()
+        k = self.lib.gr_kernel_times(self.handle, names, ms, cnt, n)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def reset_kernel_times(self) -> None:
+        self.lib.gr_reset_kernel_times(self.handle)

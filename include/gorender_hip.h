@@ -1,0 +1,223 @@
+/*
+ * gorender_hip.h — C ABI of the MI355X (gfx950) render path.
+ *
+ * This is the drop-in boundary for gorender's render path: vertex transform,
+ * frustum clipping, tile binning and the edge-function rasteriser with z-buffer,
+ * flat/Gouraud shading and texturing. A Go build with the `hip` tag binds these
+ * entry points through cgo (one call per Renderer.Draw); every type here is
+ * pointer-free or a plain pointer + size, so Go memory can be passed for the
+ * duration of a call and nothing is retained after it returns.
+ *
+ * Reference interfaces replaced (file:line in maxpoletaev/gorender):
+ *   gr_create / gr_destroy      NewRenderer (renderer.go:114-164): owns device state
+ *   gr_texture_create           NewColorTexture / NewImageTexture (texture.go:28-63)
+ *   gr_mesh_create              NewMesh (mesh.go:53-69): face normals + bounding box
+ *   gr_draw                     (*Renderer).Draw (renderer.go:443-483) writing
+ *                               FrameBuffer.Pixels / ZBuffer (rasterizer.go:7-23) and TPF
+ *   gr_draw_device / gr_draw_batch   the same Draw, asynchronous, device-resident
+ *                               outputs (no reference counterpart: batching form)
+ *
+ * Output semantics: the image is the reference's sequential draw order
+ * (renderer.go:466-474): objects, then faces, then clipped fan triangles.
+ * Errors never cross the ABI as exceptions; every call returns a gr_status and
+ * gr_last_error() returns a thread-local message.
+ */
+#ifndef GORENDER_HIP_H
+#define GORENDER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_ABI_VERSION 1
+
+typedef enum {
+    GR_OK = 0,
+    GR_ERR_INVALID_ARGUMENT = 1,
+    GR_ERR_INVALID_HANDLE = 2,
+    GR_ERR_OUT_OF_MEMORY = 3,
+    GR_ERR_DEVICE = 4,
+    /* A vertex/normal index is out of range: the reference panics with
+     * "index out of range" (renderer.go:318-330). */
+    GR_ERR_INDEX_OUT_OF_RANGE = 5,
+    GR_ERR_UNSUPPORTED = 6,
+    /* Bin capacity was exceeded on an asynchronous draw; the next draw on the
+     * context allocates the reported size. gr_draw retries by itself. */
+    GR_ERR_CAPACITY = 7,
+    /* Clip polygon exceeded 9 vertices (reference: index out of range panic,
+     * clipping.go:35-40); the face is dropped and counted in gr_stats. */
+    GR_ERR_CLIP_OVERFLOW = 8,
+} gr_status;
+
+/* Renderer toggles (renderer.go:90-97). Defaults of NewRenderer
+ * (renderer.go:130-136): SHOW_FACES|BACKFACE_CULLING|LIGHTING|FRUSTUM_CLIPPING|SHOW_TEXTURES. */
+enum {
+    GR_SHOW_FACES = 1u << 0,
+    GR_BACKFACE_CULLING = 1u << 1,
+    GR_LIGHTING = 1u << 2,
+    GR_FLAT_SHADING = 1u << 3,
+    GR_FRUSTUM_CLIPPING = 1u << 4,
+    GR_SHOW_TEXTURES = 1u << 5,
+    GR_SHOW_EDGES = 1u << 6,    /* debug overlay (renderer.go:191-210): GR_ERR_UNSUPPORTED */
+    GR_SHOW_VERTICES = 1u << 7, /* debug overlay (renderer.go:212-216): GR_ERR_UNSUPPORTED */
+};
+#define GR_DEFAULT_FLAGS \
+    (GR_SHOW_FACES | GR_BACKFACE_CULLING | GR_LIGHTING | GR_FRUSTUM_CLIPPING | GR_SHOW_TEXTURES)
+
+/* TextureType (texture.go:13-17) */
+enum { GR_TEXTURE_SOLID = 0, GR_TEXTURE_IMAGE = 1, GR_TEXTURE_IMAGE_FAST = 2 };
+
+/* Texture fields (texture.go:19-26). IMAGE_FAST requires power-of-two sizes
+ * (texture.go:40-42); NewImageTexture picks it by itself, see
+ * gr_texture_kind_for_size. */
+typedef struct {
+    int32_t type;
+    int32_t width, height;
+    float scale;      /* 1.0 by default (texture.go:53), SetScale (texture.go:65-67) */
+    uint8_t color[4]; /* returned by SOLID textures */
+} gr_texture_desc;
+
+/* Mesh data in the reference's layout (mesh.go:8-26): Vec4 positions and
+ * normals, per-face index triples and per-corner UVs. Pointers are read during
+ * the call only. Optional arrays may be NULL. */
+typedef struct {
+    int32_t num_vertices;         /* >= 1 (boundingBox reads vertices[0], mesh.go:29) */
+    const float *vertices;        /* [V][4] X,Y,Z,W */
+    int32_t num_vertex_normals;   /* 0: faces are lit by face normals (renderer.go:313) */
+    const float *vertex_normals;  /* [N][4] or NULL when N == 0 */
+    int32_t num_faces;
+    const int32_t *vertex_indices; /* [F][3], each in [0, V) */
+    const int32_t *normal_indices; /* [F][3] or NULL (all zero, a Go zero Face) */
+    const float *uvs;              /* [F][3][2] or NULL (all zero) */
+    const int32_t *face_textures;  /* [F] texture handle or -1 for nil; NULL = all nil */
+    const float *face_normals;     /* [F][4] or NULL: computed as NewMesh does (mesh.go:53-60) */
+} gr_mesh_desc;
+
+/* One object of a draw: a resident mesh and its matrices, row-major [4][4]
+ * float32 as Go builds them: mvp = ((I*P)*V)*W (renderer.go:259-262) and the
+ * world matrix W (renderer.go:255). */
+typedef struct {
+    int32_t mesh;
+    int32_t reserved;
+    float mvp[16];
+    float world[16];
+} gr_object;
+
+/* Per-frame parameters. The reference hard-codes most of them
+ * (renderer.go:10-20,118-121,265,448-449); gr_frame_defaults fills those values. */
+typedef struct {
+    int32_t width, height;  /* 1..16384 */
+    uint32_t flags;         /* GR_* toggles */
+    int32_t ref_tiles;      /* 16 (parallel, main.go:19) or 1 (sequential): TPF grid only */
+    float screen[16];       /* NewScreenMatrix (matrix.go:108-118) */
+    float light[3];         /* normalised light direction (renderer.go:265) */
+    float ambient;          /* ambientStrength 0.5 (renderer.go:13) */
+    float diffuse;          /* diffuseStrength 0.5 (renderer.go:12) */
+    float znear, zfar;      /* NewFrustum planes (clipping.go:93-129) */
+    uint8_t face_color[4];  /* faceColor (renderer.go:17) */
+    uint8_t clear_color[4]; /* fb.Clear colour (renderer.go:448) */
+    uint8_t dot_color[4];   /* fb.DotGrid colour (renderer.go:449) */
+    int32_t dot_step;       /* DotGrid step, 10 (renderer.go:449); < 1 draws no dots */
+} gr_frame;
+
+typedef struct {
+    int64_t tpf;               /* Renderer.TPF (renderer.go:436-441) on the ref_tiles grid */
+    int64_t triangles;         /* post-clip triangles binned on the full-screen tile */
+    int64_t fragments_covered; /* (pixel, triangle) pairs inside the edge test */
+    int64_t pixels_shaded;     /* pixels whose final colour comes from a triangle */
+    int64_t faces_front;       /* faces passing the backface test */
+    int64_t clip_overflow;     /* faces dropped by GR_ERR_CLIP_OVERFLOW */
+    int64_t refs;              /* tile-list entries used */
+    int64_t reserved;
+} gr_stats;
+
+/* One frame of an asynchronous batch: outputs are device pointers. */
+typedef struct {
+    gr_frame frame;
+    int32_t first_object, num_objects; /* range in the batch's object array */
+    void *d_rgba;                      /* device uint8 [H][W][4] */
+    void *d_depth;                     /* device float [H][W] */
+} gr_batch_frame;
+
+/* Tunables (autotuned). Zero fields keep the current value. */
+typedef struct {
+    int32_t tile_w, tile_h;       /* raster tile in pixels: 8, 16, 32 or 64 */
+    int64_t refs_capacity;        /* tile-list entries per batch (grown on demand) */
+    int32_t raster_waves;         /* waves per raster workgroup: 1, 2, 4 */
+    int32_t reserved;
+} gr_config;
+
+typedef struct gr_context gr_context;
+
+/* ---- context ---- */
+gr_status gr_create(int32_t device, gr_context **out);
+void gr_destroy(gr_context *ctx);
+const char *gr_last_error(void);
+int32_t gr_abi_version(void);
+gr_status gr_set_config(gr_context *ctx, const gr_config *cfg);
+gr_status gr_get_config(gr_context *ctx, gr_config *cfg);
+/* Run all work on `hip_stream` (a hipStream_t; NULL = the context's own stream). */
+gr_status gr_set_stream(gr_context *ctx, void *hip_stream);
+void *gr_get_stream(gr_context *ctx);
+
+/* ---- resident assets (HBM) ---- */
+gr_status gr_texture_create(gr_context *ctx, const gr_texture_desc *desc, const uint8_t *rgba,
+                            int32_t *out_handle);
+gr_status gr_texture_destroy(gr_context *ctx, int32_t handle);
+/* GR_TEXTURE_IMAGE_FAST when both sizes are powers of two (texture.go:40-42,
+ * utils.go:7-9), else GR_TEXTURE_IMAGE. */
+int32_t gr_texture_kind_for_size(int32_t width, int32_t height);
+gr_status gr_mesh_create(gr_context *ctx, const gr_mesh_desc *desc, int32_t *out_handle);
+gr_status gr_mesh_destroy(gr_context *ctx, int32_t handle);
+/* Bytes of HBM held by a mesh (for budget-based admission). */
+int64_t gr_mesh_device_bytes(gr_context *ctx, int32_t handle);
+
+/* ---- helpers reproducing the reference's host-side set-up ----
+ * float32 math in Go's evaluation order; row-major float[16] matrices. */
+void gr_frame_defaults(gr_frame *frame, int32_t width, int32_t height); /* renderer.go:114-136,448-449 */
+void gr_matrix_multiply(const float *a, const float *b, float *out);     /* matrix.go:155-165 */
+void gr_matrix_world(const float scale[3], const float rotation[3], const float translation[3],
+                     float *out);                                         /* matrix.go:82-88 */
+void gr_matrix_view(const float eye[3], const float direction[3], const float up[3],
+                    float *out);                                          /* matrix.go:133-144 */
+void gr_matrix_perspective(float fov, float aspect, float znear, float zfar,
+                           float *out);                                   /* matrix.go:92-106 */
+void gr_matrix_screen(int32_t width, int32_t height, float *out);         /* matrix.go:108-118 */
+void gr_matrix_mvp(const float *perspective, const float *view, const float *world,
+                   float *out);                                           /* renderer.go:259-262 */
+void gr_vec3_normalize(const float in[3], float out[3]);                  /* vector.go:92-94 */
+
+/* ---- drawing ---- */
+/* Synchronous Draw into caller-owned host buffers: rgba [H][W][4], depth [H][W]
+ * (either may be NULL), stats may be NULL. */
+gr_status gr_draw(gr_context *ctx, const gr_frame *frame, const gr_object *objects,
+                  int32_t num_objects, uint8_t *rgba, float *depth, gr_stats *stats);
+/* Asynchronous Draw on the context stream into device buffers. */
+gr_status gr_draw_device(gr_context *ctx, const gr_frame *frame, const gr_object *objects,
+                         int32_t num_objects, void *d_rgba, void *d_depth);
+/* Many independent frames in one set of launches (batched form). */
+gr_status gr_draw_batch(gr_context *ctx, const gr_batch_frame *frames, int32_t num_frames,
+                        const gr_object *objects, int32_t num_objects);
+/* Wait for queued draws; returns GR_ERR_CAPACITY if one of them overflowed its
+ * bins (its output is then incomplete and must be drawn again). */
+gr_status gr_synchronize(gr_context *ctx);
+/* Stats of the frames of the last completed draw call (after gr_synchronize). */
+gr_status gr_last_stats(gr_context *ctx, gr_stats *stats, int32_t max_frames,
+                        int32_t *num_frames);
+
+/* ---- measurement ---- */
+/* When enabled, each kernel launch is bracketed by HIP events on the context
+ * stream and its duration accumulated per kernel name. */
+gr_status gr_set_kernel_timing(gr_context *ctx, int32_t enabled);
+/* Copies up to `max` entries; names are static strings. */
+int32_t gr_kernel_times(gr_context *ctx, const char **names, double *total_ms, int64_t *launches,
+                        int32_t max);
+void gr_reset_kernel_times(gr_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
