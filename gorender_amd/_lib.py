@@ -115,6 +115,10 @@ SIGNATURES = {
     "gr_draw_batch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32]),
     "gr_synchronize": (c_int32, [c_void_p]),
     "gr_last_stats": (c_int32, [c_void_p, c_void_p, c_int32, POINTER(c_int32)]),
+    "gr_device_alloc": (c_int32, [c_void_p, ctypes.c_size_t, POINTER(c_void_p)]),
+    "gr_device_free": (c_int32, [c_void_p, c_void_p]),
+    "gr_device_fill": (c_int32, [c_void_p, c_void_p, c_int32, ctypes.c_size_t]),
+    "gr_copy_to_host": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
     "gr_set_kernel_timing": (c_int32, [c_void_p, c_int32]),
     "gr_kernel_times": (c_int32, [c_void_p, POINTER(c_char_p), POINTER(c_double),
                                   POINTER(c_int64), c_int32]),

@@ -969,6 +969,35 @@ gr_status gr_last_stats(gr_context* c, gr_stats* stats, int32_t max_frames, int3
     return GR_OK;
 }
 
+gr_status gr_device_alloc(gr_context* c, size_t bytes, void** out) {
+    if (!c || !out) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    GR_HIP(hipSetDevice(c->device));
+    GR_HIP(hipMalloc(out, std::max<size_t>(bytes, 4)));
+    return GR_OK;
+}
+
+gr_status gr_device_free(gr_context* c, void* p) {
+    if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
+    if (!p) return GR_OK;
+    GR_HIP(hipStreamSynchronize(c->stream));
+    GR_HIP(hipFree(p));
+    return GR_OK;
+}
+
+gr_status gr_device_fill(gr_context* c, void* p, int32_t value, size_t bytes) {
+    if (!c || (!p && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    if (bytes) GR_HIP(hipMemsetAsync(p, value, bytes, c->stream));
+    return GR_OK;
+}
+
+gr_status gr_copy_to_host(gr_context* c, void* dst, const void* src, size_t bytes) {
+    if (!c || ((!dst || !src) && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    GR_HIP(hipStreamSynchronize(c->stream));
+    if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GR_OK;
+}
+
 gr_status gr_set_kernel_timing(gr_context* c, int32_t enabled) {
     if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
     GR_HIP(hipStreamSynchronize(c->stream));

@@ -807,6 +807,13 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
     __syncthreads();
 
     // Deferred shading of the winning fragment of each pixel, one write per pixel.
+    // Background colours are read as values and the dot test is evaluated
+    // without short-circuit branches: hipcc 7.2 (clang 22) turned the
+    // short-circuit form into a select of field addresses that dropped the
+    // y % step condition when the loop runs once per thread.
+    const uint32_t dot_color = fr.dot_color, clear_color = fr.clear_color;
+    const int step = fr.dot_step;
+    const int step1 = step >= 1 ? step : 1;
     unsigned long long shaded = 0;
     for (int i = threadIdx.x; i < NPX; i += 256) {
         int x = X0 + (i % TW), y = Y0 + (i / TW);
@@ -816,9 +823,11 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
         uint32_t color;
         float depth;
         if (seq == 0u) {
-            int s = fr.dot_step;
-            bool dot = s >= 1 && x >= s && y >= s && (x % s) == 0 && (y % s) == 0;
-            color = dot ? fr.dot_color : fr.clear_color;
+            // DotGrid (rasterizer.go:46-52): x, y in {step, 2*step, ...}
+            uint32_t dot = (uint32_t)(step >= 1) & (uint32_t)(x >= step) &
+                           (uint32_t)(y >= step) & (uint32_t)(x % step1 == 0) &
+                           (uint32_t)(y % step1 == 0);
+            color = dot ? dot_color : clear_color;
             depth = -1.0f;
         } else {
             shaded++;

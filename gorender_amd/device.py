@@ -187,6 +187,14 @@ class Context:
         _lib.check(self.lib.gr_last_stats(self.handle, arr, max_frames, ctypes.byref(n)))
         return [arr[i].as_dict() for i in range(n.value)]
 
+    # ---- device memory -----------------------------------------------------
+    def alloc(self, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self, nbytes)
+
+    def frame_buffers(self, width: int, height: int) -> Tuple["DeviceBuffer", "DeviceBuffer"]:
+        """Device colour (H*W*4 bytes) and depth (H*W floats) buffers."""
+        return self.alloc(width * height * 4), self.alloc(width * height * 4)
+
     # ---- measurement -------------------------------------------------------
     def set_kernel_timing(self, on: bool) -> None:
         _lib.check(self.lib.gr_set_kernel_timing(self.handle, 1 if on else 0))
@@ -201,3 +209,35 @@ class Context:
 
     def reset_kernel_times(self) -> None:
         self.lib.gr_reset_kernel_times(self.handle)
+
+
+class DeviceBuffer:
+    """Device memory owned by a context (for the asynchronous draw forms)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _lib.check(ctx.lib.gr_device_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def fill(self, value: int) -> None:
+        _lib.check(self.ctx.lib.gr_device_fill(self.ctx.handle, self.ptr, value, self.nbytes))
+
+    def to_numpy(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _lib.check(self.ctx.lib.gr_copy_to_host(self.ctx.handle, out.ctypes.data_as(
+            ctypes.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.handle:
+            _lib.check(self.ctx.lib.gr_device_free(self.ctx.handle, self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -208,6 +208,15 @@ gr_status gr_synchronize(gr_context *ctx);
 gr_status gr_last_stats(gr_context *ctx, gr_stats *stats, int32_t max_frames,
                         int32_t *num_frames);
 
+/* ---- device memory for the asynchronous forms (no reference counterpart:
+ * a Go caller of gr_draw_device / gr_draw_batch owns its output buffers) ---- */
+gr_status gr_device_alloc(gr_context *ctx, size_t bytes, void **out);
+gr_status gr_device_free(gr_context *ctx, void *ptr);
+/* Fill `bytes` bytes with `value` on the context stream (asynchronous). */
+gr_status gr_device_fill(gr_context *ctx, void *ptr, int32_t value, size_t bytes);
+/* Wait for the context stream, then copy device -> host. */
+gr_status gr_copy_to_host(gr_context *ctx, void *dst, const void *src, size_t bytes);
+
 /* ---- measurement ---- */
 /* When enabled, each kernel launch is bracketed by HIP events on the context
  * stream and its duration accumulated per kernel name. */

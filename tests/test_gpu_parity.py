@@ -105,24 +105,24 @@ def test_ref_tiles_one(ctx):
 
 def test_batch_equals_single(ctx):
     """gr_draw_batch of several uneven frames == each frame drawn alone."""
-    import torch
     import scene_cases as C
     frames = [C.sphere(seed=11, w=64, h=48), C.overlapping_objects(w=100, h=70),
               C.cube_scene(w=33, h=65), C.clip_space_soup(n=200, seed=9)]
     handles = [ctx.upload_scene(s)[0] for s in frames]
     outs = []
     for s in frames:
-        outs.append((torch.empty((s.frame.height, s.frame.width, 4), dtype=torch.uint8,
-                                 device="cuda"),
-                     torch.empty((s.frame.height, s.frame.width), dtype=torch.float32,
-                                 device="cuda")))
-    torch.cuda.synchronize()
-    ctx.draw_batch(frames, handles, [(o[0].data_ptr(), o[1].data_ptr()) for o in outs])
+        rg, dp = ctx.frame_buffers(s.frame.width, s.frame.height)
+        rg.fill(0xA5)  # every pixel must be written by the draw
+        dp.fill(0x5A)
+        outs.append((rg, dp))
+    ctx.draw_batch(frames, handles, [(o[0].ptr, o[1].ptr) for o in outs])
     ctx.synchronize()
     stats = ctx.last_stats()
     for k, s in enumerate(frames):
         want = refcpu.draw(s)
-        got = (outs[k][0].cpu().numpy(), outs[k][1].cpu().numpy(), stats[k])
+        h, w = s.frame.height, s.frame.width
+        got = (outs[k][0].to_numpy(np.uint8, (h, w, 4)),
+               outs[k][1].to_numpy(np.float32, (h, w)), stats[k])
         assert_same(s, got, want)
 
 
