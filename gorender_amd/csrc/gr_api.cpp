@@ -77,14 +77,14 @@ struct gr_context {
     DevBuf d_textures;
     bool tex_dirty = true;
 
-    DevBuf objs, frames, clip, nint, vis, rrec, srec, wrec, tiles, zero32, zero64, wgcount,
-        refs, out_rgba, out_depth, ctrl;
+    // batch: frames | objects | stats | counters | frame_overflow,
+    // uploaded with one copy per draw (the counters as zeros).
+    DevBuf batch, rrec, srec, wrec, tile_counts, tiles, wgcount, refs, out_rgba, out_depth, ctrl;
     Staging ring[4];
     int ring_i = 0;
-    void* h_stats = nullptr;  // pinned: stats of the last batch
-    size_t h_stats_cap = 0;
     void* h_ctrl = nullptr;   // pinned: sticky control words
     int32_t last_frames = 0;
+    const unsigned long long* last_stats = nullptr;  // device stats of the last draw
 
     gr_config cfg{32, 16, 1 << 22, 4, 0};
 
@@ -254,6 +254,8 @@ float go_max(float a, float b) {
     return std::signbit(a) ? b : a;
 }
 
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
 // The whole pipeline for a batch of frames. Objects must be laid out frame by
 // frame; outputs are device pointers.
 gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
@@ -265,8 +267,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     std::vector<gr::DevFrame> frames(nf);
     std::vector<gr::DevObject> dobj;
     dobj.reserve(no);
-    size_t clip_n = 0, nint_n = 0;
-    int vblocks = 0, fblocks = 0, tiles = 0;
+    int fblocks = 0, tiles = 0;
     size_t slots = 0;
     for (int f = 0; f < nf; f++) {
         const gr_frame& fr = bf[f].frame;
@@ -321,32 +322,22 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             if (o.gouraud && !mr.ni_valid)
                 return fail(GR_ERR_INDEX_OUT_OF_RANGE,
                             "normal index out of range under Gouraud shading");
-            o.vtx_block = vblocks;
-            vblocks += (mr.dm.V + 255) / 256;
-            o.nrm_block = vblocks;
-            if (o.gouraud) vblocks += (mr.dm.N + 255) / 256;
-            o.end_block = vblocks;
             o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
             o.face_block = fblocks;
             o.seq_block = seq;
             fblocks += o.face_blocks;
             seq += o.face_blocks;
-            o.clip = (float4*)(uintptr_t)(clip_n * sizeof(float4));  // offsets, patched below
-            clip_n += mr.dm.V;
-            o.nint = (float*)(uintptr_t)(nint_n * sizeof(float));
-            if (o.gouraud) nint_n += mr.dm.N;
-            o.vis = (int32_t*)(uintptr_t)(dobj.size() * sizeof(int32_t));
             dobj.push_back(o);
         }
         d.face_blocks = fblocks - d.face_block_begin;
+        if ((size_t)d.face_blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
+            return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
         d.rrec = (gr::RasterRec*)(uintptr_t)(slots * sizeof(gr::RasterRec));
         d.srec = (gr::ShadeRec*)(uintptr_t)(slots * sizeof(gr::ShadeRec));
         d.wrec = (gr::WideRec*)(uintptr_t)(slots * sizeof(gr::WideRec));
         slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
-        if ((size_t)d.face_blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
-            return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
     }
-    int nobj = (int)dobj.size();
+    const int nobj = (int)dobj.size();
 
     // Texture descriptor table.
     if (c->tex_dirty) {
@@ -359,22 +350,28 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         c->tex_dirty = false;
     }
 
-    unsigned long long refs_cap = (unsigned long long)std::max<int64_t>(c->cfg.refs_capacity, 1 << 16);
+    const unsigned long long refs_cap =
+        (unsigned long long)std::max<int64_t>(c->cfg.refs_capacity, 1 << 16);
+    const size_t off_objs = align256((size_t)nf * sizeof(gr::DevFrame));
+    const size_t off_stats = off_objs + align256((size_t)nobj * sizeof(gr::DevObject));
+    const size_t off_counters = off_stats + align256((size_t)nf * gr::kNumStats * 8);
+    const size_t off_over = off_counters + align256(sizeof(gr::DevBatchCounters));
+    const size_t batch_bytes = off_over + align256((size_t)nf * 4);
+
     gr_status st;
-    if ((st = ensure(c, c->objs, nobj * sizeof(gr::DevObject)))) return st;
-    if ((st = ensure(c, c->frames, nf * sizeof(gr::DevFrame)))) return st;
-    if ((st = ensure(c, c->clip, clip_n * sizeof(float4)))) return st;
-    if ((st = ensure(c, c->nint, nint_n * sizeof(float)))) return st;
-    if ((st = ensure(c, c->vis, nobj * sizeof(int32_t)))) return st;
+    if ((st = ensure(c, c->batch, batch_bytes))) return st;
     if ((st = ensure(c, c->rrec, slots * sizeof(gr::RasterRec)))) return st;
     if ((st = ensure(c, c->srec, slots * sizeof(gr::ShadeRec)))) return st;
     if ((st = ensure(c, c->wrec, slots * sizeof(gr::WideRec)))) return st;
-    // tiles buffer: counts | overflow (zeroed) then offsets | cursor
-    size_t z32 = (size_t)tiles + nf;
-    if ((st = ensure(c, c->zero32, z32 * sizeof(uint32_t)))) return st;
+    // Tile counts are zero between draws (k_raster re-zeroes the ones it used);
+    // a new allocation starts zeroed.
+    if (c->tile_counts.cap < (size_t)tiles * 4) {
+        void* old = c->tile_counts.p;
+        if ((st = ensure(c, c->tile_counts, (size_t)tiles * 4))) return st;
+        if (c->tile_counts.p != old)
+            GR_HIP(hipMemsetAsync(c->tile_counts.p, 0, c->tile_counts.cap, c->stream));
+    }
     if ((st = ensure(c, c->tiles, 2 * (size_t)tiles * sizeof(uint32_t)))) return st;
-    size_t z64 = (size_t)nf * gr::kNumStats + 1;
-    if ((st = ensure(c, c->zero64, z64 * sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
     if ((st = ensure(c, c->refs, refs_cap * sizeof(uint32_t)))) return st;
     if (!c->ctrl.p) {
@@ -382,67 +379,80 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
     }
 
-    // Patch offsets into device pointers.
-    for (auto& o : dobj) {
-        o.clip = (float4*)((char*)c->clip.p + (uintptr_t)o.clip);
-        o.nint = (float*)((char*)c->nint.p + (uintptr_t)o.nint);
-        o.vis = (int32_t*)((char*)c->vis.p + (uintptr_t)o.vis);
-    }
+    // Offsets -> device pointers, then one upload of the whole batch header.
     for (auto& d : frames) {
         d.rrec = (gr::RasterRec*)((char*)c->rrec.p + (uintptr_t)d.rrec);
         d.srec = (gr::ShadeRec*)((char*)c->srec.p + (uintptr_t)d.srec);
         d.wrec = (gr::WideRec*)((char*)c->wrec.p + (uintptr_t)d.wrec);
     }
-    if ((st = upload(c, c->objs.p, dobj.data(), nobj * sizeof(gr::DevObject)))) return st;
-    if ((st = upload(c, c->frames.p, frames.data(), nf * sizeof(gr::DevFrame)))) return st;
+    {
+        void* h;
+        int slot;
+        if ((st = staging_get(c, batch_bytes, &h, &slot))) return st;
+        std::memset(h, 0, batch_bytes);
+        std::memcpy(h, frames.data(), (size_t)nf * sizeof(gr::DevFrame));
+        if (nobj) std::memcpy((char*)h + off_objs, dobj.data(), (size_t)nobj * sizeof(gr::DevObject));
+        GR_HIP(hipMemcpyAsync(c->batch.p, h, batch_bytes, hipMemcpyHostToDevice, c->stream));
+        if ((st = staging_mark(c, slot))) return st;
+    }
+    char* bb = (char*)c->batch.p;
 
     gr::LaunchParams p{};
-    p.objs = (const gr::DevObject*)c->objs.p;
+    p.objs = (const gr::DevObject*)(bb + off_objs);
     p.nobj = nobj;
-    p.frames = (const gr::DevFrame*)c->frames.p;
+    p.frames = (const gr::DevFrame*)bb;
     p.nframes = nf;
     p.textures = (const gr::DevTexture*)c->d_textures.p;
-    p.vertex_blocks = vblocks;
     p.face_blocks = fblocks;
     p.total_tiles = tiles;
     p.tw_log2 = twl;
     p.th_log2 = thl;
-    p.tile_counts = (uint32_t*)c->zero32.p;
-    p.frame_overflow = (uint32_t*)c->zero32.p + tiles;
+    p.tile_counts = (uint32_t*)c->tile_counts.p;
     p.tile_offsets = (uint32_t*)c->tiles.p;
     p.tile_cursor = (uint32_t*)c->tiles.p + tiles;
+    p.frame_overflow = (uint32_t*)(bb + off_over);
     p.wg_count = (uint32_t*)c->wgcount.p;
     p.refs = (uint32_t*)c->refs.p;
     p.refs_capacity = refs_cap;
-    p.stats = (unsigned long long*)c->zero64.p;
-    p.refs_used = (unsigned long long*)c->zero64.p + (size_t)nf * gr::kNumStats;
+    p.stats = (unsigned long long*)(bb + off_stats);
+    p.counters = (gr::DevBatchCounters*)(bb + off_counters);
     p.ctrl = (gr::DevControl*)c->ctrl.p;
-    p.zero_u32 = (uint32_t*)c->zero32.p;
-    p.zero_u32_n = (int)z32;
-    p.zero_u64 = (unsigned long long*)c->zero64.p;
-    p.zero_u64_n = (int)z64;
 
-    timed_launch(c, "k_vertex", [&] { gr::launch_vertex(p, c->stream); });
     timed_launch(c, "k_face", [&] { gr::launch_face(p, c->stream); });
     timed_launch(c, "k_bin_scan", [&] { gr::launch_bin_scan(p, c->stream); });
     timed_launch(c, "k_bin_fill", [&] { gr::launch_bin_fill(p, c->stream); });
     bool ok = true;
     timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
-    if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
-    GR_HIP(hipGetLastError());
-
-    // Stats of this batch to pinned host memory.
-    size_t sbytes = (size_t)nf * gr::kNumStats * sizeof(unsigned long long);
-    if (c->h_stats_cap < sbytes) {
-        GR_HIP(hipStreamSynchronize(c->stream));
-        if (c->h_stats) GR_HIP(hipHostFree(c->h_stats));
-        c->h_stats = nullptr;
-        GR_HIP(hipHostMalloc(&c->h_stats, sbytes, hipHostMallocDefault));
-        c->h_stats_cap = sbytes;
+    hipError_t le = hipGetLastError();
+    if (!ok || le != hipSuccess) {
+        // Leave the tile counters zeroed for the next draw.
+        (void)hipMemsetAsync(c->tile_counts.p, 0, c->tile_counts.cap, c->stream);
+        if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
+        return fail(GR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(le));
     }
-    GR_HIP(hipMemcpyAsync(c->h_stats, p.stats, sbytes, hipMemcpyDeviceToHost, c->stream));
     c->last_frames = nf;
+    c->last_stats = p.stats;
     return GR_OK;
+}
+
+// Copy the stats of the last draw (device) to `out` after waiting for it.
+gr_status read_stats(gr_context* c, std::vector<unsigned long long>& out) {
+    out.assign((size_t)c->last_frames * gr::kNumStats, 0ull);
+    if (!c->last_stats || out.empty()) return GR_OK;
+    GR_HIP(hipStreamSynchronize(c->stream));
+    GR_HIP(hipMemcpy(out.data(), c->last_stats, out.size() * 8, hipMemcpyDeviceToHost));
+    return GR_OK;
+}
+
+void to_gr_stats(const unsigned long long* q, gr_stats* s) {
+    s->tpf = (int64_t)q[gr::kStatTpf];
+    s->triangles = (int64_t)q[gr::kStatTriangles];
+    s->fragments_covered = (int64_t)q[gr::kStatCovered];
+    s->pixels_shaded = (int64_t)q[gr::kStatShaded];
+    s->faces_front = (int64_t)q[gr::kStatFront];
+    s->clip_overflow = (int64_t)q[gr::kStatClipOverflow];
+    s->refs = (int64_t)q[gr::kStatRefs];
+    s->reserved = 0;
 }
 
 // Wait, read and clear the sticky overflow words.
@@ -533,9 +543,9 @@ void gr_destroy(gr_context* c) {
     for (auto& m : c->meshes) free_mesh(m);
     for (auto& t : c->textures)
         if (t.texels) (void)hipFree(t.texels);
-    DevBuf* bufs[] = {&c->d_textures, &c->objs, &c->frames, &c->clip, &c->nint, &c->vis,
-                      &c->rrec, &c->srec, &c->wrec, &c->tiles, &c->zero32, &c->zero64,
-                      &c->wgcount, &c->refs, &c->out_rgba, &c->out_depth, &c->ctrl};
+    DevBuf* bufs[] = {&c->d_textures, &c->batch, &c->rrec, &c->srec, &c->wrec,
+                      &c->tile_counts, &c->tiles, &c->wgcount, &c->refs, &c->out_rgba,
+                      &c->out_depth, &c->ctrl};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& s : c->ring) {
@@ -547,7 +557,6 @@ void gr_destroy(gr_context* c) {
         (void)hipEventDestroy(t.b);
     }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
-    if (c->h_stats) (void)hipHostFree(c->h_stats);
     if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -660,37 +669,31 @@ gr_status gr_mesh_create(gr_context* c, const gr_mesh_desc* d, int32_t* out) {
         m.N = N;
         m.F = F;
         gr_status st;
-        // SoA positions; W kept only when some W != 1.
+        // Packed X,Y,Z positions (one 12-byte gather per face corner); W kept
+        // only when some W != 1.
         {
-            std::vector<float> x(V), y(V), z(V), w(V);
+            std::vector<float> xyz(3 * (size_t)V), w(V);
             bool w1 = true;
             for (int32_t i = 0; i < V; i++) {
-                x[i] = d->vertices[4 * i];
-                y[i] = d->vertices[4 * i + 1];
-                z[i] = d->vertices[4 * i + 2];
-                w[i] = d->vertices[4 * i + 3];
+                for (int k = 0; k < 3; k++) xyz[3 * (size_t)i + k] = d->vertices[4 * (size_t)i + k];
+                w[i] = d->vertices[4 * (size_t)i + 3];
                 uint32_t wb;
                 std::memcpy(&wb, &w[i], 4);
                 w1 = w1 && wb == 0x3F800000u;
             }
             void* p;
-            if ((st = upload_array(c, mr, x.data(), 4 * (size_t)V, &p))) goto failed;
-            m.px = (const float*)p;
-            if ((st = upload_array(c, mr, y.data(), 4 * (size_t)V, &p))) goto failed;
-            m.py = (const float*)p;
-            if ((st = upload_array(c, mr, z.data(), 4 * (size_t)V, &p))) goto failed;
-            m.pz = (const float*)p;
+            if ((st = upload_array(c, mr, xyz.data(), 12 * (size_t)V, &p))) goto failed;
+            m.pos = (const float*)p;
             if (!w1) {
                 if ((st = upload_array(c, mr, w.data(), 4 * (size_t)V, &p))) goto failed;
                 m.pw = (const float*)p;
             }
             // boundingBox (mesh.go:28-51)
-            float mn[3] = {x[0], y[0], z[0]}, mx[3] = {x[0], y[0], z[0]};
+            float mn[3] = {xyz[0], xyz[1], xyz[2]}, mx[3] = {xyz[0], xyz[1], xyz[2]};
             for (int32_t i = 0; i < V; i++) {
-                const float q[3] = {x[i], y[i], z[i]};
                 for (int k = 0; k < 3; k++) {
-                    mn[k] = go_min(mn[k], q[k]);
-                    mx[k] = go_max(mx[k], q[k]);
+                    mn[k] = go_min(mn[k], xyz[3 * (size_t)i + k]);
+                    mx[k] = go_max(mx[k], xyz[3 * (size_t)i + k]);
                 }
             }
             int k = 0;
@@ -705,24 +708,19 @@ gr_status gr_mesh_create(gr_context* c, const gr_mesh_desc* d, int32_t* out) {
                     }
         }
         if (N > 0) {
-            std::vector<float> x(N), y(N), z(N), w(N);
+            std::vector<float> xyz(3 * (size_t)N), w(N);
             bool w1 = true;
             for (int32_t i = 0; i < N; i++) {
-                x[i] = d->vertex_normals[4 * i];
-                y[i] = d->vertex_normals[4 * i + 1];
-                z[i] = d->vertex_normals[4 * i + 2];
-                w[i] = d->vertex_normals[4 * i + 3];
+                for (int k = 0; k < 3; k++)
+                    xyz[3 * (size_t)i + k] = d->vertex_normals[4 * (size_t)i + k];
+                w[i] = d->vertex_normals[4 * (size_t)i + 3];
                 uint32_t wb;
                 std::memcpy(&wb, &w[i], 4);
                 w1 = w1 && wb == 0x3F800000u;
             }
             void* p;
-            if ((st = upload_array(c, mr, x.data(), 4 * (size_t)N, &p))) goto failed;
-            m.nx = (const float*)p;
-            if ((st = upload_array(c, mr, y.data(), 4 * (size_t)N, &p))) goto failed;
-            m.ny = (const float*)p;
-            if ((st = upload_array(c, mr, z.data(), 4 * (size_t)N, &p))) goto failed;
-            m.nz = (const float*)p;
+            if ((st = upload_array(c, mr, xyz.data(), 12 * (size_t)N, &p))) goto failed;
+            m.nrm = (const float*)p;
             if (!w1) {
                 if ((st = upload_array(c, mr, w.data(), 4 * (size_t)N, &p))) goto failed;
                 m.nw = (const float*)p;
@@ -804,7 +802,7 @@ gr_status gr_mesh_create(gr_context* c, const gr_mesh_desc* d, int32_t* out) {
                 }
                 mr.allocs.push_back(fnp);
                 mr.bytes += 16 * (int64_t)F;
-                gr::launch_face_normals(m.px, m.py, m.pz, m.vi, F, (float*)fnp, c->stream);
+                gr::launch_face_normals(m.pos, m.vi, F, (float*)fnp, c->stream);
                 if (hipStreamSynchronize(c->stream) != hipSuccess || hipGetLastError() != hipSuccess) {
                     st = fail(GR_ERR_DEVICE, "face normal kernel failed");
                     goto failed;
@@ -931,15 +929,9 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
             if (rgba) GR_HIP(hipMemcpy(rgba, c->out_rgba.p, npx * 4, hipMemcpyDeviceToHost));
             if (depth) GR_HIP(hipMemcpy(depth, c->out_depth.p, npx * 4, hipMemcpyDeviceToHost));
             if (stats) {
-                const unsigned long long* s = (const unsigned long long*)c->h_stats;
-                stats->tpf = (int64_t)s[gr::kStatTpf];
-                stats->triangles = (int64_t)s[gr::kStatTriangles];
-                stats->fragments_covered = (int64_t)s[gr::kStatCovered];
-                stats->pixels_shaded = (int64_t)s[gr::kStatShaded];
-                stats->faces_front = (int64_t)s[gr::kStatFront];
-                stats->clip_overflow = (int64_t)s[gr::kStatClipOverflow];
-                stats->refs = (int64_t)s[gr::kStatRefs];
-                stats->reserved = 0;
+                std::vector<unsigned long long> sv;
+                if ((st = read_stats(c, sv))) return st;
+                to_gr_stats(sv.data(), stats);
             }
             return GR_OK;
         }
@@ -951,21 +943,12 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
 
 gr_status gr_last_stats(gr_context* c, gr_stats* stats, int32_t max_frames, int32_t* nf) {
     if (!c || !stats) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    std::vector<unsigned long long> sv;
+    gr_status st = read_stats(c, sv);
+    if (st) return st;
     int32_t n = std::min(max_frames, c->last_frames);
-    const unsigned long long* s = (const unsigned long long*)c->h_stats;
-    for (int32_t f = 0; f < n && s; f++) {
-        const unsigned long long* q = s + (size_t)f * gr::kNumStats;
-        stats[f].tpf = (int64_t)q[gr::kStatTpf];
-        stats[f].triangles = (int64_t)q[gr::kStatTriangles];
-        stats[f].fragments_covered = (int64_t)q[gr::kStatCovered];
-        stats[f].pixels_shaded = (int64_t)q[gr::kStatShaded];
-        stats[f].faces_front = (int64_t)q[gr::kStatFront];
-        stats[f].clip_overflow = (int64_t)q[gr::kStatClipOverflow];
-        stats[f].refs = (int64_t)q[gr::kStatRefs];
-        stats[f].reserved = 0;
-    }
-    if (nf) *nf = s ? n : 0;
+    for (int32_t f = 0; f < n; f++) to_gr_stats(sv.data() + (size_t)f * gr::kNumStats, &stats[f]);
+    if (nf) *nf = n;
     return GR_OK;
 }
 

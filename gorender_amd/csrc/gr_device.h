@@ -38,14 +38,10 @@ enum : uint32_t {
 struct DevMesh {
     int32_t V, N, F;
     int32_t mesh_tex;        // texture used by every face when face_tex == nullptr
-    const float* px;         // SoA positions
-    const float* py;
-    const float* pz;
-    const float* pw;         // nullptr when every W == 1
-    const float* nx;         // SoA vertex normals (N entries)
-    const float* ny;
-    const float* nz;
-    const float* nw;         // nullptr when every W == 1
+    const float* pos;        // [V*3] packed X,Y,Z (one 12-byte gather per corner)
+    const float* pw;         // [V] W, nullptr when every W == 1
+    const float* nrm;        // [N*3] packed vertex normals
+    const float* nw;         // [N] W, nullptr when every W == 1
     const int32_t* vi;       // [F*3]
     const int32_t* ni;       // [F*3]; nullptr when identical to vi
     const float* uv_vert;    // [V*2] per-vertex UVs, or nullptr
@@ -61,15 +57,9 @@ struct DevObject {
     float world[16];
     int32_t frame;          // index in the batch
     int32_t gouraud;        // vertex-normal lighting in use
-    int32_t vtx_block;      // first k_vertex block of this object
-    int32_t nrm_block;      // first normal block (after the vertex blocks)
-    int32_t end_block;      // one past the last k_vertex block
     int32_t face_block;     // first k_face block (batch-global)
     int32_t face_blocks;    // number of k_face blocks
     int32_t seq_block;      // first face block index within the frame (draw order)
-    float4* clip;           // scratch [V] clip-space positions
-    float* nint;            // scratch [N] Gouraud intensity per normal
-    int32_t* vis;           // scratch: box visibility
     int32_t pad;
 };
 
@@ -138,6 +128,12 @@ enum : int32_t {
     kStatRefs,
     kStatReserved,
     kNumStats
+};
+
+// Per-batch counters, uploaded as zeros with the descriptors of every draw.
+struct DevBatchCounters {
+    unsigned long long refs_used;  // tile-list entries allocated so far
+    unsigned long long pad;
 };
 
 // Sticky control words (cleared by the host when it reads them): set when a

@@ -1,15 +1,18 @@
 // gfx950 kernels of the render path.
 //
-//   k_vertex    MVP transform of positions (SoA, coalesced), World transform +
-//               4-D normalise + Lambert term of vertex normals, box visibility.
-//   k_face      per face: backface test, lighting, frustum clipping, perspective
-//               divide + screen transform, full-screen bin test, raster set-up,
-//               tile counting. One workgroup = 256 faces = 1792 ordered slots.
-//   k_bin_scan  per frame: exclusive scan of tile counts -> tile-list offsets.
-//   k_bin_fill  per face workgroup: scatter slots into tile lists.
-//   k_raster    one workgroup per screen tile: visibility in LDS with a 64-bit
-//               (depth, draw-sequence) max key, then one deferred shading pass
-//               and a single coalesced write of colour + depth.
+//   k_face      per face: MVP transform of the three corners, backface test,
+//               lighting, frustum clipping (wave-cooperative Sutherland-Hodgman
+//               with ballot/popcount compaction), perspective divide + screen
+//               transform, full-screen bin test, raster set-up. One workgroup
+//               = 256 faces = 1792 slots in draw order. The workgroup then
+//               counts its tile references in LDS and adds them to the global
+//               tile counts.
+//   k_bin_scan  one workgroup per frame: tile counts -> tile-list offsets.
+//   k_bin_fill  per face workgroup: scatter slots into tile lists, with one
+//               global atomic per distinct tile per workgroup.
+//   k_raster    one workgroup per screen tile: triangles staged in LDS,
+//               visibility with a 64-bit (depth, draw-sequence) max key in LDS,
+//               one deferred shading pass and a single write of colour + depth.
 //
 // Every float32 expression keeps the reference's evaluation order; the build
 // uses -ffp-contract=off and correctly rounded division/sqrt.
@@ -79,7 +82,8 @@ __device__ __forceinline__ float4 xform(const float* m, float x, float y, float 
     return r;
 }
 
-// vector.go:115-125 (Vec4.Length + Divide), then renderer.go:331/336 Lambert term.
+// vector.go:115-125 (Vec4.Length + Divide, W dropped by ToVec3), then the
+// Lambert term of renderer.go:331 / 336-337.
 __device__ __forceinline__ float lambert(float4 n, const float* light, float ambient,
                                          float diffuse) {
     float len = __builtin_sqrtf(n.x * n.x + n.y * n.y + n.z * n.z + n.w * n.w);
@@ -88,6 +92,7 @@ __device__ __forceinline__ float lambert(float4 n, const float* light, float amb
     return ambient + d * diffuse;
 }
 
+// Clip-space vertex with its attributes.
 struct CV {
     float x, y, z, w, u, v, i;
 };
@@ -128,46 +133,6 @@ __device__ __forceinline__ CV intersect(const float4& P, const float4& N, const 
     r.u = A.u + (B.u - A.u) * f;
     r.v = A.v + (B.v - A.v) * f;
     return r;
-}
-
-// One Sutherland-Hodgman pass (clipping.go:199-232) from `in` (n vertices) into
-// `out`; returns the new count, or -1 when a 10th vertex would be added.
-__device__ __noinline__ int clip_pass(int p, float zn, float zf, const CV* in, int n, CV* out) {
-    float4 P, N;
-    plane_of(p, zn, zf, P, N);
-    int m = 0;
-    for (int b = 0; b < n; b++) {
-        int a = (b + 1 == n) ? 0 : b + 1;
-        CV A = in[a], B = in[b];
-        bool ia = inside(P, N, A);
-        if (ia) {
-            if (!inside(P, N, B)) {
-                if (m >= kMaxClip) return -1;
-                out[m++] = intersect(P, N, A, B);
-            }
-            if (m >= kMaxClip) return -1;
-            out[m++] = A;
-        } else if (inside(P, N, B)) {
-            if (m >= kMaxClip) return -1;
-            out[m++] = intersect(P, N, A, B);
-        }
-    }
-    return m;
-}
-
-// ClipTriangle (clipping.go:167-236) for a triangle already in poly[0..2];
-// the final polygon is left in poly. Returns its vertex count, 0 when clipped
-// away, -1 on overflow.
-__device__ __noinline__ int clip_general(CV* poly, float zn, float zf) {
-    CV tmp[kMaxClip];
-    int n = 3;
-    for (int p = 0; p < 6; p += 2) {
-        n = clip_pass(p, zn, zf, poly, n, tmp);
-        if (n <= 0) return n;
-        n = clip_pass(p + 1, zn, zf, tmp, n, poly);
-        if (n <= 0) return n;
-    }
-    return n;
 }
 
 // Inside bits of a vertex for the 6 planes (bit p = inside plane p).
@@ -213,98 +178,173 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-// ---------------------------------------------------------------------------
-// k_vertex
-// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) { return (1ull << lane) - 1ull; }
 
-__global__ __launch_bounds__(256) void k_vertex(const DevObject* __restrict__ objs, int nobj,
-                                                const DevFrame* __restrict__ frames,
-                                                int vertex_blocks, uint32_t* __restrict__ zero_u32,
-                                                int zero_u32_n,
-                                                unsigned long long* __restrict__ zero_u64,
-                                                int zero_u64_n) {
-    int b = blockIdx.x;
-    if (b >= vertex_blocks) {
-        // Clear the batch counters (tile counts, per-frame flags, stats).
-        int zb = b - vertex_blocks;
-        int nz = gridDim.x - vertex_blocks;
-        for (int i = zb * 256 + threadIdx.x; i < zero_u32_n; i += nz * 256) zero_u32[i] = 0u;
-        for (int i = zb * 256 + threadIdx.x; i < zero_u64_n; i += nz * 256) zero_u64[i] = 0ull;
-        return;
-    }
-    int oi = last_le(nobj, b, [&](int i) { return objs[i].vtx_block; });
-    const DevObject& o = objs[oi];
-    const DevMesh& m = o.m;
-    if (b < o.nrm_block) {
-        int v = (b - o.vtx_block) * 256 + threadIdx.x;
-        if (v < m.V) {
-            float w = m.pw ? m.pw[v] : 1.0f;
-            o.clip[v] = xform(o.mvp, m.px[v], m.py[v], m.pz[v], w);
-        }
-        if (b == o.vtx_block && threadIdx.x == 0) {
-            // renderer.go:268-275 + clipping.go:131-154 on the transformed box.
-            const DevFrame& fr = frames[o.frame];
-            int vis = kBoxInside;
-            for (int p = 0; p < 6 && vis == kBoxInside; p++) {
-                float4 P, N;
-                plane_of(p, fr.znear, fr.zfar, P, N);
-                float np = N.x * P.x + N.y * P.y + N.z * P.z + N.w * P.w;
-                int outside = 0;
-                for (int k = 0; k < 8; k++) {
-                    float4 c = xform(o.mvp, m.bbox[k][0], m.bbox[k][1], m.bbox[k][2],
-                                     m.bbox[k][3]);
-                    float d = N.x * c.x + N.y * c.y + N.z * c.z + N.w * c.w;
-                    if (d - np > 0.f) outside++;
-                }
-                if (outside == 8) vis = kBoxOutside;
-                else if (outside > 0) vis = kBoxIntersect;
-            }
-            *o.vis = vis;
-        }
-    } else {
-        int n = (b - o.nrm_block) * 256 + threadIdx.x;
-        if (n < m.N) {
-            const DevFrame& fr = frames[o.frame];
-            float w = m.nw ? m.nw[n] : 1.0f;
-            float4 t = xform(o.world, m.nx[n], m.ny[n], m.nz[n], w);
-            o.nint[n] = lambert(t, fr.light, fr.ambient, fr.diffuse);
-        }
-    }
+__device__ __forceinline__ CV shfl_cv(const CV& a, int src) {
+    CV r;
+    r.x = __shfl(a.x, src, 64);
+    r.y = __shfl(a.y, src, 64);
+    r.z = __shfl(a.z, src, 64);
+    r.w = __shfl(a.w, src, 64);
+    r.u = __shfl(a.u, src, 64);
+    r.v = __shfl(a.v, src, 64);
+    r.i = __shfl(a.i, src, 64);
+    return r;
 }
 
 // ---------------------------------------------------------------------------
-// k_face
+// Wave-cooperative ClipTriangle (clipping.go:167-236)
 // ---------------------------------------------------------------------------
 
-// Exclusive block scan of small counts (256 threads = 4 waves).
-__device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total) {
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
+// Every lane passes the same triangle (a, b, c). Per plane pass, lane k owns
+// edge (k, (k+1) % n) of the polygon and contributes [X, A], [A], [X] or []
+// exactly as the reference's loop body; ballots give each lane its output
+// position. The polygon ping-pongs between buf0 and buf1 in LDS (one pair per
+// wave) and ends in buf0 after the 6 passes. Returns the final vertex count
+// (uniform), 0 when clipped away, -1 when a pass would exceed 9 vertices.
+__device__ int wave_clip(CV* buf0, CV* buf1, const CV& a, const CV& b, const CV& c, float zn,
+                         float zf, int lane) {
+    if (lane == 0) buf0[0] = a;
+    if (lane == 1) buf0[1] = b;
+    if (lane == 2) buf0[2] = c;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int n = 3;
+    CV* in = buf0;
+    CV* out = buf1;
+    for (int p = 0; p < 6; p++) {
+        float4 P, N;
+        plane_of(p, zn, zf, P, N);
+        const bool act = lane < n;
+        const int ai = act ? (lane + 1 == n ? 0 : lane + 1) : 0;
+        const CV A = in[ai];
+        const CV B = in[act ? lane : 0];
+        const bool ia = act && inside(P, N, A);
+        const bool ib = act && inside(P, N, B);
+        const bool one = ia || ib;     // at least one output vertex
+        const bool two = ia && !ib;    // [X, A]
+        const uint64_t b1 = __ballot(one), b2 = __ballot(two);
+        const uint64_t lt = lanemask_lt(lane);
+        const int pos = __popcll(b1 & lt) + __popcll(b2 & lt);
+        const int m = __popcll(b1) + __popcll(b2);
+        if (m > kMaxClip) return -1;
+        if (m == 0) return 0;
+        if (ia != ib) {
+            CV X = intersect(P, N, A, B);
+            out[pos] = X;
+            if (two) out[pos + 1] = A;
+        } else if (one) {
+            out[pos] = A;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        n = m;
+        CV* t = in;
+        in = out;
+        out = t;
     }
-    if (lane == 63) lds4[wave] = x;
-    __syncthreads();
-    int base = 0;
-    total = 0;
-    for (int w = 0; w < 4; w++) {
-        int t = lds4[w];
-        if (w < wave) base += t;
-        total += t;
-    }
-    return base + x - v;
+    return n;
 }
+
+// ---------------------------------------------------------------------------
+// Per-lane clipping of a triangle that straddles one plane
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ CV sel(bool c, const CV& a, const CV& b) {
+    CV r;
+    r.x = c ? a.x : b.x;
+    r.y = c ? a.y : b.y;
+    r.z = c ? a.z : b.z;
+    r.w = c ? a.w : b.w;
+    r.u = c ? a.u : b.u;
+    r.v = c ? a.v : b.v;
+    r.i = c ? a.i : b.i;
+    return r;
+}
+
+// Append v at position m of q[0..3] with register selects (no dynamic index).
+__device__ __forceinline__ void append4(CV q[4], int& m, const CV& v) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = sel(m == k, v, q[k]);
+    m++;
+}
+
+// ClipTriangle for a triangle whose three vertices are inside the planes before
+// p and straddle plane p (at least one vertex inside). Passes before p only
+// rotate the triangle left by one (clipping.go:204-224 with every vertex
+// inside), so plane p sees it rotated by p mod 3; the pass over p runs the
+// reference's edge loop. Each later plane is then either passed by every
+// vertex (a pure rotation), failed by every vertex (the reference returns 0 at
+// that pass, clipping.go:227-229), or straddled: only the last case needs the
+// full clipper. Returns -1 for it, 0 when clipped away, otherwise the vertex
+// count (3 or 4) of the final polygon in q.
+__device__ int clip_one_plane(int p, const CV& c0, const CV& c1, const CV& c2, float zn,
+                              float zf, CV q[4]) {
+    const int r = p % 3;
+    const CV R0 = sel(r == 0, c0, sel(r == 1, c1, c2));
+    const CV R1 = sel(r == 0, c1, sel(r == 1, c2, c0));
+    const CV R2 = sel(r == 0, c2, sel(r == 1, c0, c1));
+    float4 P, N;
+    plane_of(p, zn, zf, P, N);
+    const bool i0 = inside(P, N, R0), i1 = inside(P, N, R1), i2 = inside(P, N, R2);
+    q[0] = q[1] = q[2] = q[3] = R0;
+    int m = 0;
+    // edges b = 0, 1, 2 with A = R[(b+1) % 3], B = R[b]
+    const CV As[3] = {R1, R2, R0}, Bs[3] = {R0, R1, R2};
+    const bool iA[3] = {i1, i2, i0}, iB[3] = {i0, i1, i2};
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        if (iA[b]) {
+            if (!iB[b]) append4(q, m, intersect(P, N, As[b], Bs[b]));
+            append4(q, m, As[b]);
+        } else if (iB[b]) {
+            append4(q, m, intersect(P, N, As[b], Bs[b]));
+        }
+    }
+    int rot = 0;
+#pragma unroll
+    for (int pl = 0; pl < 6; pl++) {
+        if (pl <= p) continue;
+        float4 P2, N2;
+        plane_of(pl, zn, zf, P2, N2);
+        int nin = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) nin += (k < m && inside(P2, N2, q[k])) ? 1 : 0;
+        if (nin == 0) return 0;
+        if (nin != m) return -1;
+        rot++;
+    }
+    // `rot` more left rotations of an m-gon.
+    rot %= m;
+    CV t[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        // t[k] = q[(k + rot) % m]
+        const int s3 = (k + rot) % 3, s4 = (k + rot) % 4;
+        const int s = m == 3 ? s3 : s4;
+        t[k] = sel(s == 0, q[0], sel(s == 1, q[1], sel(s == 2, q[2], q[3])));
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) q[k] = t[k];
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle set-up
+// ---------------------------------------------------------------------------
 
 struct FaceStats {
     unsigned long long tpf, tris, front, overflow;
 };
 
 // Writes the set-up of one post-clip triangle into its slot (renderer.go:361-394
-// then the per-triangle part of rasterizer.go:98-137).
-__device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, const CV& c,
-                              int tex, uint32_t slot, int tw_log2, int th_log2,
-                              uint32_t* __restrict__ tile_counts, FaceStats& st) {
+// then the per-triangle part of rasterizer.go:98-137). Returns the TPF
+// references in the low 16 bits and 1 << 16 when the triangle was binned.
+__device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, const CV& c,
+                                  int tex, uint32_t slot) {
+    uint32_t tpf = 0, tris = 0;
     const CV vv[3] = {a, b, c};
     float sx[3], sy[3], zw[3];
 #pragma unroll
@@ -322,7 +362,7 @@ __device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, cons
     // renderer.go:226-244 against the reference TPF grid.
     for (int t = 0; t < fr.ref_tiles; t++) {
         const float* r = fr.ref_tile_rect[t];
-        if (fmaxx >= r[0] && fminx <= r[2] && fmaxy >= r[1] && fminy <= r[3]) st.tpf++;
+        if (fmaxx >= r[0] && fminx <= r[2] && fmaxy >= r[1] && fminy <= r[3]) tpf++;
     }
     RasterRec rr;
     rr.bx0 = 1;
@@ -337,7 +377,7 @@ __device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, cons
     float W = (float)fr.W, H = (float)fr.H;
     bool binned = fmaxx >= 0.f && fminx <= W && fmaxy >= 0.f && fminy <= H;
     if (binned) {
-        st.tris++;
+        tris = 1;
         int64_t ix[3], iy[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
@@ -384,8 +424,8 @@ __device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, cons
                 C64[e] = C;
                 csum = wadd(csum, C);
             }
-            // Order of the reference sum (fx12 + fx20) + fx01 is irrelevant for
-            // wrapping integers.
+            // The pixel sum fx12 + fx20 + fx01 is this constant (the A and B
+            // terms cancel); wrapping addition makes the order irrelevant.
             rr.sf = (float)csum;
             if (fast) {
 #pragma unroll
@@ -396,8 +436,6 @@ __device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, cons
                 }
             } else {
                 rr.wide = 1;
-#pragma unroll
-                for (int e = 0; e < 9; e++) rr.e[e] = 0;
                 WideRec wr;
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
@@ -422,43 +460,166 @@ __device__ void emit_triangle(const DevFrame& fr, const CV& a, const CV& b, cons
             sr.tex = tex;
             sr.pad[0] = sr.pad[1] = 0;
             fr.srec[slot] = sr;
-            int tx0 = (int)mnx >> tw_log2, tx1 = (int)mxx >> tw_log2;
-            int ty0 = (int)mny >> th_log2, ty1 = (int)mxy >> th_log2;
-            for (int ty = ty0; ty <= ty1; ty++)
-                for (int tx = tx0; tx <= tx1; tx++)
-                    atomicAdd(&tile_counts[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
         }
     }
     fr.rrec[slot] = rr;
+    return tpf | (tris << 16);
+}
+
+__device__ __forceinline__ void add_emit(FaceStats& st, uint32_t e) {
+    st.tpf += e & 0xFFFFu;
+    st.tris += e >> 16;
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup tile binning helpers
+// ---------------------------------------------------------------------------
+
+constexpr int kLdsTiles = 1024;  // LDS tile counters per workgroup
+
+struct TileRect {
+    int tx0, tx1, ty0, ty1;
+};
+
+// Tile range of a raster record (empty: tx0 > tx1).
+__device__ __forceinline__ TileRect rec_tiles(const RasterRec& r, int twl, int thl) {
+    TileRect t;
+    if (r.bx0 > r.bx1) {
+        t.tx0 = 1;
+        t.tx1 = 0;
+        t.ty0 = 1;
+        t.ty1 = 0;
+    } else {
+        t.tx0 = (int)r.bx0 >> twl;
+        t.tx1 = (int)r.bx1 >> twl;
+        t.ty0 = (int)r.by0 >> thl;
+        t.ty1 = (int)r.by1 >> thl;
+    }
+    return t;
+}
+
+// Union of the tile ranges of the workgroup's records [slot0, slot0 + cnt),
+// into lrect (LDS, 4 ints) by LDS atomics. Ends with a barrier.
+__device__ void block_tile_rect(const RasterRec* rrec, uint32_t slot0, uint32_t cnt, int twl,
+                                int thl, int* lrect) {
+    if (threadIdx.x == 0) {
+        lrect[0] = 0x7FFFFFFF;
+        lrect[1] = -1;
+        lrect[2] = 0x7FFFFFFF;
+        lrect[3] = -1;
+    }
+    __syncthreads();
+    int a = 0x7FFFFFFF, b = -1, c = 0x7FFFFFFF, d = -1;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+        TileRect t = rec_tiles(rrec[slot0 + i], twl, thl);
+        if (t.tx0 > t.tx1) continue;
+        a = min(a, t.tx0);
+        b = max(b, t.tx1);
+        c = min(c, t.ty0);
+        d = max(d, t.ty1);
+    }
+    if (b >= 0) {
+        atomicMin(&lrect[0], a);
+        atomicMax(&lrect[1], b);
+        atomicMin(&lrect[2], c);
+        atomicMax(&lrect[3], d);
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// k_face
+// ---------------------------------------------------------------------------
+
+// Exclusive block scan of small counts (256 threads = 4 waves).
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds4[wave] = x;
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int w = 0; w < 4; w++) {
+        int t = lds4[w];
+        if (w < wave) base += t;
+        total += t;
+    }
+    return base + x - v;
+}
+
+// renderer.go:268-275 + clipping.go:131-154: lanes 0..7 transform one box
+// corner each; per plane the outside corners are counted by ballot. Uniform.
+__device__ int box_visibility(const DevObject& o, const DevFrame& fr, int lane) {
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < 8)
+        c = xform(o.mvp, o.m.bbox[lane][0], o.m.bbox[lane][1], o.m.bbox[lane][2],
+                  o.m.bbox[lane][3]);
+    for (int p = 0; p < 6; p++) {
+        float4 P, N;
+        plane_of(p, fr.znear, fr.zfar, P, N);
+        float np = N.x * P.x + N.y * P.y + N.z * P.z + N.w * P.w;
+        float d = N.x * c.x + N.y * c.y + N.z * c.z + N.w * c.w;
+        int outside = __popcll(__ballot(lane < 8 && d - np > 0.f));
+        if (outside == 8) return kBoxOutside;
+        if (outside > 0) return kBoxIntersect;
+    }
+    return kBoxInside;
+}
+
+// Clip-space corner k of a face: MVP * (x, y, z, w) (renderer.go:303-304,318-320).
+__device__ __forceinline__ float4 corner(const DevObject& o, int v) {
+    const float* p = o.m.pos + 3 * (size_t)v;
+    float w = o.m.pw ? o.m.pw[v] : 1.0f;
+    return xform(o.mvp, p[0], p[1], p[2], w);
+}
+
+// Gouraud intensity of normal n: World * normal, 4-D normalise (renderer.go:308-310,328-333).
+__device__ __forceinline__ float normal_intensity(const DevObject& o, const DevFrame& fr, int n) {
+    const float* q = o.m.nrm + 3 * (size_t)n;
+    float w = o.m.nw ? o.m.nw[n] : 1.0f;
+    return lambert(xform(o.world, q[0], q[1], q[2], w), fr.light, fr.ambient, fr.diffuse);
 }
 
 __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs, int nobj,
                                               const DevFrame* __restrict__ frames,
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
-                                              unsigned long long* __restrict__ stats,
-                                              int tw_log2, int th_log2) {
+                                              unsigned long long* __restrict__ stats, int twl,
+                                              int thl) {
     __shared__ int lds_scan[4];
     __shared__ unsigned long long lds_st[4][4];
-    int b = blockIdx.x;
-    int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
+    __shared__ CV wpoly[4][2][kMaxClip];
+    __shared__ uint32_t lcnt[kLdsTiles];
+    __shared__ int lrect[4];
+
+    const int b = blockIdx.x;
+    const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
     const DevObject& o = objs[oi];
     const DevMesh& m = o.m;
     const DevFrame& fr = frames[o.frame];
-    int wg_in_obj = b - o.face_block;
-    int face = wg_in_obj * kFacesPerWg + threadIdx.x;
-    uint32_t flags = fr.flags;
-    int vis = *o.vis;
+    const int wg_in_obj = b - o.face_block;
+    const int face = wg_in_obj * kFacesPerWg + threadIdx.x;
+    const uint32_t flags = fr.flags;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int vis = box_visibility(o, fr, lane);
 
     FaceStats st = {0, 0, 0, 0};
-    int mode = 0;  // 0: nothing, 1: triangle as is, 2: clipped polygon
+    // 0: nothing, 1: emit the triangle as is, 2: full clipper in the wave,
+    // 3: one-plane polygon in q[0..nq-1]
+    int mode = 0;
     int n = 0;
-    CV poly[kMaxClip];
+    CV c0, c1, c2;
+    CV q[4];
+    int nq = 0;
     int tex = -1;
 
     if (face < m.F && vis != kBoxOutside) {
-        int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
-        float4 v0 = o.clip[i0], v1 = o.clip[i1], v2 = o.clip[i2];
+        const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
+        const float4 v0 = corner(o, i0), v1 = corner(o, i1), v2 = corner(o, i2);
         if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
             st.front++;
             float in0, in1, in2;
@@ -470,11 +631,11 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
                         n1 = m.ni[3 * face + 1];
                         n2 = m.ni[3 * face + 2];
                     }
-                    in0 = o.nint[n0];
-                    in1 = o.nint[n1];
-                    in2 = o.nint[n2];
+                    in0 = normal_intensity(o, fr, n0);
+                    in1 = normal_intensity(o, fr, n1);
+                    in2 = normal_intensity(o, fr, n2);
                 } else {
-                    const float* f = m.fn + 4 * face;
+                    const float* f = m.fn + 4 * (size_t)face;
                     float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
                     in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
                 }
@@ -490,7 +651,7 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
                 u2 = m.uv_vert[2 * i2];
                 w2 = m.uv_vert[2 * i2 + 1];
             } else if (m.uv_face) {
-                const float* uv = m.uv_face + 6 * face;
+                const float* uv = m.uv_face + 6 * (size_t)face;
                 u0 = uv[0];
                 w0 = uv[1];
                 u1 = uv[2];
@@ -499,34 +660,33 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
                 w2 = uv[5];
             }
             if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[face] : m.mesh_tex;
-            poly[0] = CV{v0.x, v0.y, v0.z, v0.w, u0, w0, in0};
-            poly[1] = CV{v1.x, v1.y, v1.z, v1.w, u1, w1, in1};
-            poly[2] = CV{v2.x, v2.y, v2.z, v2.w, u2, w2, in2};
+            c0 = CV{v0.x, v0.y, v0.z, v0.w, u0, w0, in0};
+            c1 = CV{v1.x, v1.y, v1.z, v1.w, u1, w1, in1};
+            c2 = CV{v2.x, v2.y, v2.z, v2.w, u2, w2, in2};
             if ((flags & kFrustumClipping) && vis != kBoxInside) {
-                uint32_t m0 = inside_mask(poly[0], fr.znear, fr.zfar);
-                uint32_t m1 = inside_mask(poly[1], fr.znear, fr.zfar);
-                uint32_t m2 = inside_mask(poly[2], fr.znear, fr.zfar);
-                uint32_t all_in = m0 & m1 & m2;
+                const uint32_t m0 = inside_mask(c0, fr.znear, fr.zfar);
+                const uint32_t m1 = inside_mask(c1, fr.znear, fr.zfar);
+                const uint32_t m2 = inside_mask(c2, fr.znear, fr.zfar);
+                const uint32_t all_in = m0 & m1 & m2;
                 if (all_in == 0x3Fu) {
                     // Every pass only rotates the triangle; after 6 passes it is
                     // unchanged (clipping.go:199-232).
                     mode = 1;
                     n = 1;
                 } else {
-                    int p = __builtin_ctz(~all_in & 0x3Fu);
-                    uint32_t any_in = (m0 | m1 | m2) >> p & 1u;
-                    if (!any_in) {
+                    const int p = __builtin_ctz(~all_in & 0x3Fu);
+                    if (!(((m0 | m1 | m2) >> p) & 1u)) {
                         // Earlier planes left the triangle untouched; this one
                         // rejects every vertex (polygon2.Count == 0).
                         n = 0;
                     } else {
-                        int k = clip_general(poly, fr.znear, fr.zfar);
-                        if (k < 0) {
-                            st.overflow++;
-                            n = 0;
-                        } else {
-                            mode = 2;
-                            n = k >= 3 ? k - 2 : 0;
+                        nq = clip_one_plane(p, c0, c1, c2, fr.znear, fr.zfar, q);
+                        if (nq > 0) {
+                            mode = 3;
+                            n = nq - 2;
+                        } else if (nq < 0) {
+                            nq = 0;
+                            mode = 2;  // counted below, one face at a time per wave
                         }
                     }
                 }
@@ -537,59 +697,126 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
         }
     }
 
+    // Count the fan triangles of the faces that need the full clipper.
+    CV* buf0 = wpoly[wave][0];
+    CV* buf1 = wpoly[wave][1];
+    uint64_t gen = __ballot(mode == 2);
+    for (uint64_t bits = gen; bits; bits &= bits - 1) {
+        const int L = __builtin_ctzll(bits);
+        const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
+        const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
+        if (lane == L) {
+            if (k < 0) {
+                st.overflow++;
+                mode = 0;
+                n = 0;
+            } else {
+                n = k >= 3 ? k - 2 : 0;
+            }
+        }
+    }
+
     int total;
-    int off = block_exclusive_scan(n, lds_scan, total);
-    uint32_t slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg + (uint32_t)off;
-    if (mode == 1) {
-        emit_triangle(fr, poly[0], poly[1], poly[2], tex, slot0, tw_log2, th_log2, tile_counts,
-                      st);
-    } else if (mode == 2) {
-        for (int i = 0; i < n; i++)
-            emit_triangle(fr, poly[0], poly[i + 1], poly[i + 2], tex, slot0 + i, tw_log2,
-                          th_log2, tile_counts, st);
+    const int off = block_exclusive_scan(n, lds_scan, total);
+    const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
+    const uint32_t slot0 = wg_slot0 + (uint32_t)off;
+    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0));
+    if (mode == 3) {
+        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0));
+        if (nq == 4) add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1));
+    }
+
+    // Clip again (deterministic) and emit the fan with one lane per triangle.
+    gen = __ballot(mode == 2 && n > 0);
+    for (uint64_t bits = gen; bits; bits &= bits - 1) {
+        const int L = __builtin_ctzll(bits);
+        const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
+        const int nL = __shfl(n, L, 64);
+        const int texL = __shfl(tex, L, 64);
+        const uint32_t sL = (uint32_t)__shfl((int)slot0, L, 64);
+        wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
+        if (lane < nL)
+            add_emit(st, emit_triangle(fr, buf0[0], buf0[lane + 1], buf0[lane + 2], texL, sL + lane));
     }
     if (threadIdx.x == 0) wg_count[b] = (uint32_t)total;
 
     // Block-reduce the counters, one atomic per counter per block.
-    unsigned long long s0 = wave_sum_u64(st.tpf), s1 = wave_sum_u64(st.tris);
-    unsigned long long s2 = wave_sum_u64(st.front), s3 = wave_sum_u64(st.overflow);
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        lds_st[wave][0] = s0;
-        lds_st[wave][1] = s1;
-        lds_st[wave][2] = s2;
-        lds_st[wave][3] = s3;
+    {
+        unsigned long long s0 = wave_sum_u64(st.tpf), s1 = wave_sum_u64(st.tris);
+        unsigned long long s2 = wave_sum_u64(st.front), s3 = wave_sum_u64(st.overflow);
+        if (lane == 0) {
+            lds_st[wave][0] = s0;
+            lds_st[wave][1] = s1;
+            lds_st[wave][2] = s2;
+            lds_st[wave][3] = s3;
+        }
     }
-    __syncthreads();
+    // The barrier inside block_tile_rect also publishes this block's records
+    // to all of its threads.
+    block_tile_rect(fr.rrec, wg_slot0, (uint32_t)total, twl, thl, lrect);
     if (threadIdx.x < 4) {
         unsigned long long t = lds_st[0][threadIdx.x] + lds_st[1][threadIdx.x] +
                                lds_st[2][threadIdx.x] + lds_st[3][threadIdx.x];
         const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
         if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
     }
+
+    // Tile counting: LDS counters over the block's tile rectangle, one global
+    // atomic per touched tile; very large rectangles count directly.
+    const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
+    if (rx1 >= 0) {
+        const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
+        const bool in_lds = rw * rh <= kLdsTiles;
+        if (in_lds)
+            for (int i = threadIdx.x; i < rw * rh; i += 256) lcnt[i] = 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < total; i += 256) {
+            TileRect t = rec_tiles(fr.rrec[wg_slot0 + i], twl, thl);
+            for (int ty = t.ty0; ty <= t.ty1; ty++)
+                for (int tx = t.tx0; tx <= t.tx1; tx++) {
+                    if (in_lds)
+                        atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
+                    else
+                        atomicAdd(&tile_counts[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
+                }
+        }
+        __syncthreads();
+        if (in_lds)
+            for (int i = threadIdx.x; i < rw * rh; i += 256) {
+                uint32_t v = lcnt[i];
+                if (v)
+                    atomicAdd(&tile_counts[fr.tile_begin + (ry0 + i / rw) * fr.tiles_x + rx0 +
+                                           i % rw],
+                              v);
+            }
+    }
+
 }
 
 // ---------------------------------------------------------------------------
-// k_bin_scan: tile counts -> list offsets, per frame
+// k_bin_scan: tile counts -> list offsets, one workgroup per frame
 // ---------------------------------------------------------------------------
 
+// A kernel of its own: a last-block scan inside k_face would need a device-scope
+// release in every face workgroup, which on a multi-XCD part writes back and
+// invalidates L2 each time.
 __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ frames,
                                                    uint32_t* __restrict__ tile_counts,
                                                    uint32_t* __restrict__ tile_offsets,
                                                    uint32_t* __restrict__ tile_cursor,
                                                    uint32_t* __restrict__ frame_overflow,
                                                    unsigned long long* __restrict__ stats,
-                                                   unsigned long long* __restrict__ refs_used,
+                                                   DevBatchCounters* __restrict__ counters,
                                                    DevControl* __restrict__ ctrl,
                                                    unsigned long long refs_capacity) {
     __shared__ unsigned long long wsum[16];
     __shared__ unsigned long long base_s;
-    const DevFrame& fr = frames[blockIdx.x];
-    int ntiles = fr.tiles_x * fr.tiles_y;
+    const int f = blockIdx.x;
+    const DevFrame& fr = frames[f];
+    const int ntiles = fr.tiles_x * fr.tiles_y;
     uint32_t* cnt = tile_counts + fr.tile_begin;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // Pass 1: total refs of the frame.
     unsigned long long local = 0;
     for (int i = threadIdx.x; i < ntiles; i += 1024) local += cnt[i];
     local = wave_sum_u64(local);
@@ -598,26 +825,25 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
     if (threadIdx.x == 0) {
         unsigned long long tot = 0;
         for (int w = 0; w < 16; w++) tot += wsum[w];
-        unsigned long long base = atomicAdd(refs_used, tot);
-        stats[blockIdx.x * kNumStats + kStatRefs] = tot;
+        const unsigned long long base = atomicAdd(&counters->refs_used, tot);
+        stats[f * kNumStats + kStatRefs] = tot;
         if (base + tot > refs_capacity) {
-            frame_overflow[blockIdx.x] = 1u;
+            frame_overflow[f] = 1u;
             atomicOr(&ctrl->overflow, 1u);
             atomicMax(&ctrl->refs_needed, base + tot);
         }
         base_s = base;
     }
     __syncthreads();
-    bool over = frame_overflow[blockIdx.x] != 0u;
+    const bool over = frame_overflow[f] != 0u;
     unsigned long long run = base_s;
-    // Pass 2: exclusive scan in chunks of 1024.
     for (int c = 0; c < ntiles; c += 1024) {
-        int i = c + threadIdx.x;
-        uint32_t v = i < ntiles ? cnt[i] : 0u;
+        const int i = c + threadIdx.x;
+        const uint32_t v = i < ntiles ? cnt[i] : 0u;
         unsigned long long x = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            unsigned long long y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
+        for (int s = 1; s < 64; s <<= 1) {
+            unsigned long long y = __shfl_up(x, s, 64);
+            if (lane >= s) x += y;
         }
         __syncthreads();
         if (lane == 63) wsum[wave] = x;
@@ -628,9 +854,9 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
             ctot += wsum[w];
         }
         if (i < ntiles) {
-            unsigned long long off = run + wbase + x - v;
-            tile_offsets[fr.tile_begin + i] = (uint32_t)off;
-            tile_cursor[fr.tile_begin + i] = (uint32_t)off;
+            const unsigned long long offv = run + wbase + x - v;
+            tile_offsets[fr.tile_begin + i] = (uint32_t)offv;
+            tile_cursor[fr.tile_begin + i] = (uint32_t)offv;
             if (over) cnt[i] = 0u;  // overflowed frame: raster draws background only
         }
         run += ctot;
@@ -646,26 +872,59 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
                                                   const uint32_t* __restrict__ wg_count,
                                                   const uint32_t* __restrict__ frame_overflow,
                                                   uint32_t* __restrict__ tile_cursor,
-                                                  uint32_t* __restrict__ refs, int tw_log2,
-                                                  int th_log2) {
-    int b = blockIdx.x;
-    int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
+                                                  uint32_t* __restrict__ refs, int twl,
+                                                  int thl) {
+    __shared__ uint32_t lcnt[kLdsTiles];
+    __shared__ int lrect[4];
+    const int b = blockIdx.x;
+    const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
     const DevObject& o = objs[oi];
     if (frame_overflow[o.frame]) return;
     const DevFrame& fr = frames[o.frame];
-    uint32_t cnt = wg_count[b];
-    uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
+    const uint32_t cnt = wg_count[b];
+    if (cnt == 0) return;
+    const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
+    block_tile_rect(fr.rrec, slot0, cnt, twl, thl, lrect);
+    const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
+    if (rx1 < 0) return;
+    const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
+    if (rw * rh > kLdsTiles) {
+        // Very large rectangle: one global atomic per reference.
+        for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+            TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+            for (int ty = t.ty0; ty <= t.ty1; ty++)
+                for (int tx = t.tx0; tx <= t.tx1; tx++) {
+                    const uint32_t pos =
+                        atomicAdd(&tile_cursor[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
+                    refs[pos] = slot0 + i;
+                }
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < rw * rh; i += 256) lcnt[i] = 0u;
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-        uint32_t slot = slot0 + i;
-        const RasterRec& r = fr.rrec[slot];
-        int bx0 = r.bx0, bx1 = r.bx1, by0 = r.by0, by1 = r.by1;
-        if (bx0 > bx1) continue;
-        int tx0 = bx0 >> tw_log2, tx1 = bx1 >> tw_log2;
-        int ty0 = by0 >> th_log2, ty1 = by1 >> th_log2;
-        for (int ty = ty0; ty <= ty1; ty++)
-            for (int tx = tx0; tx <= tx1; tx++) {
-                uint32_t pos = atomicAdd(&tile_cursor[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
-                refs[pos] = slot;
+        TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+        for (int ty = t.ty0; ty <= t.ty1; ty++)
+            for (int tx = t.tx0; tx <= t.tx1; tx++)
+                atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
+    }
+    __syncthreads();
+    // One global reservation per touched tile; lcnt becomes the write cursor.
+    for (int i = threadIdx.x; i < rw * rh; i += 256) {
+        const uint32_t v = lcnt[i];
+        if (v)
+            lcnt[i] = atomicAdd(&tile_cursor[fr.tile_begin + (ry0 + i / rw) * fr.tiles_x + rx0 +
+                                            i % rw],
+                                v);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+        TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+        for (int ty = t.ty0; ty <= t.ty1; ty++)
+            for (int tx = t.tx0; tx <= t.tx1; tx++) {
+                const uint32_t pos = atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
+                refs[pos] = slot0 + i;
             }
     }
 }
@@ -760,51 +1019,69 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
 
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ frames, int nframes,
-                                                const uint32_t* __restrict__ tile_counts,
+                                                uint32_t* __restrict__ tile_counts,
                                                 const uint32_t* __restrict__ tile_offsets,
                                                 const uint32_t* __restrict__ refs,
                                                 const DevTexture* __restrict__ textures,
                                                 unsigned long long* __restrict__ stats) {
     constexpr int NPX = TW * TH;
+    constexpr int CHUNK = 256;  // triangles staged in LDS at a time
     __shared__ unsigned long long key[NPX];
+    __shared__ RasterRec lrec[CHUNK];
+    __shared__ uint32_t lslot[CHUNK];
     __shared__ unsigned long long red[4][2];
-    int g = blockIdx.x;
-    int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
+    const int g = blockIdx.x;
+    const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
     const DevFrame& fr = frames[fi];
-    int t = g - fr.tile_begin;
-    int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = g - fr.tile_begin;
+    const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
     for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
-    __syncthreads();
 
-    uint32_t cnt = tile_counts[g], off = tile_offsets[g];
+    const uint32_t cnt = tile_counts[g], off = tile_offsets[g];
     unsigned long long covered = 0;
-    for (uint32_t i = wave; i < cnt; i += 4) {
-        uint32_t slot = refs[off + i];
-        const RasterRec r = fr.rrec[slot];
-        const WideRec* wr = r.wide ? &fr.wrec[slot] : nullptr;
-        int x0 = max((int)r.bx0, X0), x1 = min((int)r.bx1, X0 + TW - 1);
-        int y0 = max((int)r.by0, Y0), y1 = min((int)r.by1, Y0 + TH - 1);
-        int bw = x1 - x0 + 1;
-        int lpr_log2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
-        if (lpr_log2 > 6) lpr_log2 = 6;
-        int lpr = 1 << lpr_log2, rows = 64 >> lpr_log2;
-        int lx = lane & (lpr - 1), ly = lane >> lpr_log2;
-        for (int yb = y0; yb <= y1; yb += rows) {
-            int y = yb + ly;
-            for (int xb = x0; xb <= x1; xb += lpr) {
-                int x = xb + lx;
-                if (y > y1 || x > x1) continue;
-                EdgeEval e = eval_pixel(r, wr, x, y);
-                if (!e.in) continue;
-                covered++;
-                if (e.zrec >= -1.0f)
-                    atomicMax(&key[(y - Y0) * TW + (x - X0)], depth_key(e.zrec, slot + 1u));
+    for (uint32_t base = 0; base < cnt; base += CHUNK) {
+        const uint32_t n = min(cnt - base, (uint32_t)CHUNK);
+        // Stage the chunk: slots (one coalesced read), then the 64-byte raster
+        // records as four 16-byte pieces per record.
+        __syncthreads();
+        if (threadIdx.x < n) lslot[threadIdx.x] = refs[off + base + threadIdx.x];
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < 4 * n; p += 256) {
+            const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[p >> 2]]);
+            reinterpret_cast<uint4*>(&lrec[p >> 2])[p & 3] = src[p & 3];
+        }
+        __syncthreads();
+        for (uint32_t i = wave; i < n; i += 4) {
+            const uint32_t slot = lslot[i];
+            const RasterRec r = lrec[i];
+            const WideRec* wr = r.wide ? &fr.wrec[slot] : nullptr;
+            const int x0 = max((int)r.bx0, X0), x1 = min((int)r.bx1, X0 + TW - 1);
+            const int y0 = max((int)r.by0, Y0), y1 = min((int)r.by1, Y0 + TH - 1);
+            const int bw = x1 - x0 + 1;
+            int lpr_log2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
+            if (lpr_log2 > 6) lpr_log2 = 6;
+            const int lpr = 1 << lpr_log2, rows = 64 >> lpr_log2;
+            const int lx = lane & (lpr - 1), ly = lane >> lpr_log2;
+            for (int yb = y0; yb <= y1; yb += rows) {
+                const int y = yb + ly;
+                for (int xb = x0; xb <= x1; xb += lpr) {
+                    const int x = xb + lx;
+                    if (y > y1 || x > x1) continue;
+                    EdgeEval e = eval_pixel(r, wr, x, y);
+                    if (!e.in) continue;
+                    covered++;
+                    if (e.zrec >= -1.0f)
+                        atomicMax(&key[(y - Y0) * TW + (x - X0)], depth_key(e.zrec, slot + 1u));
+                }
             }
         }
     }
     __syncthreads();
+    // Tile counts are zero between draws: the face stage of the next draw
+    // accumulates into them again.
+    if (threadIdx.x == 0 && cnt) tile_counts[g] = 0u;
 
     // Deferred shading of the winning fragment of each pixel, one write per pixel.
     // Background colours are read as values and the dot test is evaluated
@@ -816,22 +1093,22 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
     const int step1 = step >= 1 ? step : 1;
     unsigned long long shaded = 0;
     for (int i = threadIdx.x; i < NPX; i += 256) {
-        int x = X0 + (i % TW), y = Y0 + (i / TW);
+        const int x = X0 + (i % TW), y = Y0 + (i / TW);
         if (x >= fr.W || y >= fr.H) continue;
-        unsigned long long k = key[i];
-        uint32_t seq = (uint32_t)k;
+        const unsigned long long k = key[i];
+        const uint32_t seq = (uint32_t)k;
         uint32_t color;
         float depth;
         if (seq == 0u) {
             // DotGrid (rasterizer.go:46-52): x, y in {step, 2*step, ...}
-            uint32_t dot = (uint32_t)(step >= 1) & (uint32_t)(x >= step) &
-                           (uint32_t)(y >= step) & (uint32_t)(x % step1 == 0) &
-                           (uint32_t)(y % step1 == 0);
+            const uint32_t dot = (uint32_t)(step >= 1) & (uint32_t)(x >= step) &
+                                 (uint32_t)(y >= step) & (uint32_t)(x % step1 == 0) &
+                                 (uint32_t)(y % step1 == 0);
             color = dot ? dot_color : clear_color;
             depth = -1.0f;
         } else {
             shaded++;
-            uint32_t slot = seq - 1u;
+            const uint32_t slot = seq - 1u;
             const RasterRec r = fr.rrec[slot];
             const ShadeRec s = fr.srec[slot];
             const WideRec* wr = r.wide ? &fr.wrec[slot] : nullptr;
@@ -844,7 +1121,7 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
             color = color_intensity(c, inten);
             depth = e.zrec;
         }
-        size_t idx = (size_t)y * fr.W + x;
+        const size_t idx = (size_t)y * fr.W + x;
         ((uint32_t*)fr.rgba)[idx] = color;
         fr.depth[idx] = depth;
     }
@@ -857,8 +1134,8 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
     }
     __syncthreads();
     if (threadIdx.x < 2) {
-        unsigned long long s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                               red[3][threadIdx.x];
+        const unsigned long long s = red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                     red[2][threadIdx.x] + red[3][threadIdx.x];
         if (s) atomicAdd(&stats[fi * kNumStats + (threadIdx.x ? kStatShaded : kStatCovered)], s);
     }
 }
@@ -867,42 +1144,33 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
 // Mesh set-up: face normals as NewMesh computes them (mesh.go:53-60)
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_face_normals(const float* __restrict__ px,
-                                                      const float* __restrict__ py,
-                                                      const float* __restrict__ pz,
+__global__ __launch_bounds__(256) void k_face_normals(const float* __restrict__ pos,
                                                       const int32_t* __restrict__ vi, int F,
                                                       float* __restrict__ fn) {
     int f = blockIdx.x * 256 + threadIdx.x;
     if (f >= F) return;
-    int a = vi[3 * f], b = vi[3 * f + 1], c = vi[3 * f + 2];
-    float ax = px[b] - px[a], ay = py[b] - py[a], az = pz[b] - pz[a];
-    float bx = px[c] - px[a], by = py[c] - py[a], bz = pz[c] - pz[a];
+    const float* a = pos + 3 * (size_t)vi[3 * f];
+    const float* b = pos + 3 * (size_t)vi[3 * f + 1];
+    const float* c = pos + 3 * (size_t)vi[3 * f + 2];
+    float ax = b[0] - a[0], ay = b[1] - a[1], az = b[2] - a[2];
+    float bx = c[0] - a[0], by = c[1] - a[1], bz = c[2] - a[2];
     float nx = ay * bz - az * by;
     float ny = az * bx - ax * bz;
     float nz = ax * by - ay * bx;
     float len = __builtin_sqrtf(nx * nx + ny * ny + nz * nz);
-    fn[4 * f + 0] = nx / len;
-    fn[4 * f + 1] = ny / len;
-    fn[4 * f + 2] = nz / len;
-    fn[4 * f + 3] = 1.0f;
+    fn[4 * (size_t)f + 0] = nx / len;
+    fn[4 * (size_t)f + 1] = ny / len;
+    fn[4 * (size_t)f + 2] = nz / len;
+    fn[4 * (size_t)f + 3] = 1.0f;
 }
 
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-void launch_face_normals(const float* px, const float* py, const float* pz, const int32_t* vi,
-                         int F, float* fn, hipStream_t s) {
+void launch_face_normals(const float* pos, const int32_t* vi, int F, float* fn, hipStream_t s) {
     if (F <= 0) return;
-    hipLaunchKernelGGL(k_face_normals, dim3((F + 255) / 256), dim3(256), 0, s, px, py, pz, vi, F,
-                       fn);
-}
-
-void launch_vertex(const LaunchParams& p, hipStream_t s) {
-    int zero_blocks = 64;
-    hipLaunchKernelGGL(k_vertex, dim3(p.vertex_blocks + zero_blocks), dim3(256), 0, s, p.objs,
-                       p.nobj, p.frames, p.vertex_blocks, p.zero_u32, p.zero_u32_n, p.zero_u64,
-                       p.zero_u64_n);
+    hipLaunchKernelGGL(k_face_normals, dim3((F + 255) / 256), dim3(256), 0, s, pos, vi, F, fn);
 }
 
 void launch_face(const LaunchParams& p, hipStream_t s) {
@@ -913,7 +1181,7 @@ void launch_face(const LaunchParams& p, hipStream_t s) {
 
 void launch_bin_scan(const LaunchParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_scan, dim3(p.nframes), dim3(1024), 0, s, p.frames, p.tile_counts,
-                       p.tile_offsets, p.tile_cursor, p.frame_overflow, p.stats, p.refs_used,
+                       p.tile_offsets, p.tile_cursor, p.frame_overflow, p.stats, p.counters,
                        p.ctrl, p.refs_capacity);
 }
 

@@ -8,17 +8,18 @@
 
 namespace gr {
 
+// Device memory of one batch. The host uploads frames, objects and the zeroed
+// counters (stats, frame_overflow, batch counters) with one copy.
 struct LaunchParams {
     const DevObject* objs;
     int nobj;
     const DevFrame* frames;
     int nframes;
     const DevTexture* textures;
-    int vertex_blocks;       // k_vertex blocks covering all objects
     int face_blocks;         // k_face / k_bin_fill blocks
     int total_tiles;         // raster workgroups
     int tw_log2, th_log2;    // raster tile size
-    uint32_t* tile_counts;   // [total_tiles]
+    uint32_t* tile_counts;   // [total_tiles]; zero between draws (k_raster re-zeroes)
     uint32_t* tile_offsets;  // [total_tiles]
     uint32_t* tile_cursor;   // [total_tiles]
     uint32_t* frame_overflow;// [nframes]
@@ -26,18 +27,11 @@ struct LaunchParams {
     uint32_t* refs;          // [refs_capacity]
     unsigned long long refs_capacity;
     unsigned long long* stats;  // [nframes * kNumStats]
-    unsigned long long* refs_used;  // batch allocation cursor of refs
+    DevBatchCounters* counters;
     DevControl* ctrl;
-    // zeroed by k_vertex before anything else runs
-    uint32_t* zero_u32;
-    int zero_u32_n;
-    unsigned long long* zero_u64;
-    int zero_u64_n;
 };
 
-void launch_face_normals(const float* px, const float* py, const float* pz, const int32_t* vi,
-                         int F, float* fn, hipStream_t s);
-void launch_vertex(const LaunchParams& p, hipStream_t s);
+void launch_face_normals(const float* pos, const int32_t* vi, int F, float* fn, hipStream_t s);
 void launch_face(const LaunchParams& p, hipStream_t s);
 void launch_bin_scan(const LaunchParams& p, hipStream_t s);
 void launch_bin_fill(const LaunchParams& p, hipStream_t s);
