@@ -1017,6 +1017,33 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
     return r | (g << 8) | (b << 16) | (c & 0xFF000000u);
 }
 
+// Exact depth of `count` queued fragments of one wave (rasterizer.go:149-156),
+// one per lane, and the (depth, sequence) max into the tile key.
+template <int TW>
+__device__ __forceinline__ void frag_depth(const uint32_t* qpix, const int32_t* qf12,
+                                           const int32_t* qf20, uint32_t head, uint32_t count,
+                                           const RasterRec* lrec, const uint32_t* lslot,
+                                           unsigned long long* key, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if ((uint32_t)lane < count) {
+        const uint32_t k = (head + (uint32_t)lane) & 127u;
+        const uint32_t pk = qpix[k];
+        const int32_t f12 = qf12[k], f20 = qf20[k];
+        const uint32_t i = pk >> 12, px = (pk >> 6) & 63u, py = pk & 63u;
+        const float4 d = *reinterpret_cast<const float4*>(&lrec[i]);  // sf, z0, z1, z2
+        const float alpha = (float)f12 / d.x;
+        const float beta = (float)f20 / d.x;
+        const float gamma = 1.0f - alpha - beta;
+        const float zrec = -(alpha / d.y + beta / d.z + gamma / d.w);
+        if (zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(zrec, lslot[i] + 1u));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int TW, int TH>
 __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ frames, int nframes,
                                                 uint32_t* __restrict__ tile_counts,
@@ -1029,6 +1056,9 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
     __shared__ unsigned long long key[NPX];
     __shared__ RasterRec lrec[CHUNK];
     __shared__ uint32_t lslot[CHUNK];
+    __shared__ uint32_t qpix[4][128];
+    __shared__ int32_t qf12[4][128];
+    __shared__ int32_t qf20[4][128];
     __shared__ unsigned long long red[4][2];
     const int g = blockIdx.x;
     const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
@@ -1053,10 +1083,12 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
             reinterpret_cast<uint4*>(&lrec[p >> 2])[p & 3] = src[p & 3];
         }
         __syncthreads();
+        // Per wave: the integer edge test runs over pixel blocks of each triangle
+        // and inside fragments are appended to an LDS ring by ballot; the exact
+        // depth (five divisions) then runs on full waves of fragments.
+        uint32_t qhead = 0, qtail = 0;  // wave-uniform ring positions
         for (uint32_t i = wave; i < n; i += 4) {
-            const uint32_t slot = lslot[i];
             const RasterRec r = lrec[i];
-            const WideRec* wr = r.wide ? &fr.wrec[slot] : nullptr;
             const int x0 = max((int)r.bx0, X0), x1 = min((int)r.bx1, X0 + TW - 1);
             const int y0 = max((int)r.by0, Y0), y1 = min((int)r.by1, Y0 + TH - 1);
             const int bw = x1 - x0 + 1;
@@ -1064,18 +1096,55 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
             if (lpr_log2 > 6) lpr_log2 = 6;
             const int lpr = 1 << lpr_log2, rows = 64 >> lpr_log2;
             const int lx = lane & (lpr - 1), ly = lane >> lpr_log2;
+            if (r.wide) {
+                // Coordinates beyond the int32 range: direct int64 evaluation.
+                const uint32_t slot = lslot[i];
+                const WideRec* wr = &fr.wrec[slot];
+                for (int yb = y0; yb <= y1; yb += rows) {
+                    const int y = yb + ly;
+                    for (int xb = x0; xb <= x1; xb += lpr) {
+                        const int x = xb + lx;
+                        if (y > y1 || x > x1) continue;
+                        EdgeEval e = eval_pixel(r, wr, x, y);
+                        if (!e.in) continue;
+                        covered++;
+                        if (e.zrec >= -1.0f)
+                            atomicMax(&key[(y - Y0) * TW + (x - X0)],
+                                      depth_key(e.zrec, slot + 1u));
+                    }
+                }
+                continue;
+            }
             for (int yb = y0; yb <= y1; yb += rows) {
                 const int y = yb + ly;
                 for (int xb = x0; xb <= x1; xb += lpr) {
                     const int x = xb + lx;
-                    if (y > y1 || x > x1) continue;
-                    EdgeEval e = eval_pixel(r, wr, x, y);
-                    if (!e.in) continue;
-                    covered++;
-                    if (e.zrec >= -1.0f)
-                        atomicMax(&key[(y - Y0) * TW + (x - X0)], depth_key(e.zrec, slot + 1u));
+                    const int32_t f01 = r.e[0] * x + r.e[1] * y + r.e[2];
+                    const int32_t f12 = r.e[3] * x + r.e[4] * y + r.e[5];
+                    const int32_t f20 = r.e[6] * x + r.e[7] * y + r.e[8];
+                    const bool in = y <= y1 && x <= x1 && f01 < 0 && f12 < 0 && f20 < 0;
+                    const uint64_t mask = __ballot(in);
+                    if (in) {
+                        const uint32_t k = (qtail + __popcll(mask & lanemask_lt(lane))) & 127u;
+                        qpix[wave][k] = (i << 12) | ((uint32_t)(x - X0) << 6) | (uint32_t)(y - Y0);
+                        qf12[wave][k] = f12;
+                        qf20[wave][k] = f20;
+                        covered++;
+                    }
+                    qtail += (uint32_t)__popcll(mask);
+                    if (qtail - qhead >= 64u) {
+                        frag_depth<TW>(qpix[wave], qf12[wave], qf20[wave], qhead, 64u, lrec,
+                                       lslot, key, lane);
+                        qhead += 64u;
+                    }
                 }
             }
+        }
+        while (qtail != qhead) {
+            const uint32_t cnt_q = min(qtail - qhead, 64u);
+            frag_depth<TW>(qpix[wave], qf12[wave], qf20[wave], qhead, cnt_q, lrec, lslot, key,
+                           lane);
+            qhead += cnt_q;
         }
     }
     __syncthreads();
