@@ -40,6 +40,9 @@ def parse():
                     help="short run for rocprofv3: no CPU baseline, no latency loops")
     ap.add_argument("--mode", choices=["both", "batched", "plain"], default="both",
                     help="forms to run (rocprofv3 runs profile one form at a time)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run k_copy over this many MiB (known-size kernel that the "
+                         "HBM byte counters are calibrated against)")
     return ap.parse_args()
 
 
@@ -138,6 +141,16 @@ def main():
         ctx.synchronize()
         kt_plain = ctx.kernel_times()
         ctx.set_kernel_timing(False)
+
+    if args.calib_copy_mib > 0:
+        nb = args.calib_copy_mib << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        src.fill(0x3C)
+        for _ in range(4):
+            dst.copy_from(src)
+        ctx.synchronize()
+        src.free()
+        dst.free()
 
     lat_dev, lat_host = [], []
     if not args.profile:

@@ -118,6 +118,9 @@ SIGNATURES = {
     "gr_device_alloc": (c_int32, [c_void_p, ctypes.c_size_t, POINTER(c_void_p)]),
     "gr_device_free": (c_int32, [c_void_p, c_void_p]),
     "gr_device_fill": (c_int32, [c_void_p, c_void_p, c_int32, ctypes.c_size_t]),
+    "gr_device_copy": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
+    "gr_check_exact_division": (c_int32, [c_void_p, ctypes.c_uint64, ctypes.c_uint64,
+                                          POINTER(ctypes.c_uint64), POINTER(ctypes.c_float)]),
     "gr_copy_to_host": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
     "gr_set_kernel_timing": (c_int32, [c_void_p, c_int32]),
     "gr_kernel_times": (c_int32, [c_void_p, POINTER(c_char_p), POINTER(c_double),

@@ -617,6 +617,8 @@ gr_status gr_texture_create(gr_context* c, const gr_texture_desc* d, const uint8
         if (d->type == GR_TEXTURE_IMAGE || d->type == GR_TEXTURE_IMAGE_FAST) {
             if (d->width < 1 || d->height < 1 || !rgba)
                 return fail(GR_ERR_INVALID_ARGUMENT, "image texture needs size and texels");
+            if ((int64_t)d->width * d->height > ((int64_t)1 << 28))
+                return fail(GR_ERR_INVALID_ARGUMENT, "texture larger than 2^28 texels");
             if (d->type == GR_TEXTURE_IMAGE_FAST && !(is_pow2(d->width) && is_pow2(d->height)))
                 return fail(GR_ERR_INVALID_ARGUMENT, "IMAGE_FAST needs power-of-two sizes");
             size_t bytes = (size_t)d->width * d->height * 4;
@@ -971,6 +973,37 @@ gr_status gr_device_free(gr_context* c, void* p) {
 gr_status gr_device_fill(gr_context* c, void* p, int32_t value, size_t bytes) {
     if (!c || (!p && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
     if (bytes) GR_HIP(hipMemsetAsync(p, value, bytes, c->stream));
+    return GR_OK;
+}
+
+gr_status gr_device_copy(gr_context* c, void* dst, const void* src, size_t bytes) {
+    if (!c || ((!dst || !src) && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    GR_HIP(hipSetDevice(c->device));
+    timed_launch(c, "k_copy", [&] { gr::launch_copy(dst, src, bytes, c->stream); });
+    GR_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+gr_status gr_check_exact_division(gr_context* c, uint64_t count, uint64_t seed,
+                                  uint64_t* mismatches, float* first_mismatch) {
+    if (!c || !mismatches) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    GR_HIP(hipSetDevice(c->device));
+    GR_HIP(hipStreamSynchronize(c->stream));
+    void* buf = nullptr;
+    GR_HIP(hipMalloc(&buf, 32));
+    struct Free {
+        void* p;
+        ~Free() { (void)hipFree(p); }
+    } guard{buf};
+    GR_HIP(hipMemsetAsync(buf, 0, 32, c->stream));
+    gr::launch_check_div(count, seed, (unsigned long long*)buf, (float*)((char*)buf + 16),
+                         c->stream);
+    GR_HIP(hipGetLastError());
+    unsigned char host[32];
+    GR_HIP(hipMemcpyAsync(host, buf, 32, hipMemcpyDeviceToHost, c->stream));
+    GR_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(mismatches, host, 8);
+    if (first_mismatch) std::memcpy(first_mismatch, host + 16, 16);
     return GR_OK;
 }
 

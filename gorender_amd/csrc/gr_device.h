@@ -92,16 +92,18 @@ struct DevFrame {
     struct WideRec* wrec;
 };
 
-// Per-triangle raster set-up, 64 bytes: what the visibility pass reads. The
-// four divisors come first so one aligned 16-byte read fetches them.
+// Per-triangle raster set-up, 80 bytes: what the visibility pass reads. The
+// four divisors and their reciprocals come first (two aligned 16-byte reads).
 struct alignas(16) RasterRec {
     float sf;                    // float32(fx12+fx20+fx01), constant per triangle
     float z0, z1, z2;            // clip-space W of the vertices (renderer.go:369)
+    float rsf, rz0, rz1, rz2;    // RN(1/divisor) when every divisor allows the
+                                 // reciprocal-based exact quotient, else all 0
     int16_t bx0, bx1, by0, by1;  // clamped pixel bounding box; bx0 > bx1 = empty
     uint32_t wide;               // 1: coordinates beyond kFastCoord, use WideRec
     int32_t e[9];                // A01,B01,C01', A12,B12,C12', A20,B20,C20'
 };
-static_assert(sizeof(RasterRec) == 64, "RasterRec is one 64-byte line");
+static_assert(sizeof(RasterRec) == 80, "RasterRec is five 16-byte pieces");
 
 // Per-triangle shading inputs, 48 bytes.
 struct alignas(16) ShadeRec {

@@ -69,6 +69,32 @@ __device__ __forceinline__ int64_t wmul(int64_t a, int64_t b) {
 }
 
 // ---------------------------------------------------------------------------
+// Exact float32 division from a correctly rounded reciprocal
+// ---------------------------------------------------------------------------
+
+// |d| in [2^-40, 2^41): quotients a/d of numerators in [2^-70, 2^70] then keep
+// every intermediate of div_rcp in the normal range.
+__device__ __forceinline__ bool rcp_range(float d) {
+    const uint32_t e = (__float_as_uint(d) >> 23) & 0xFFu;
+    return e - (127u - 40u) <= 80u;
+}
+
+// RN(a / d) given y = RN(1 / d). q0 = RN(a*y) is within 1.5 ulp of a/d; one
+// residual correction brings it within 1 ulp; with y the correctly rounded
+// reciprocal and q within 1 ulp, the residual a - d*q is exact and
+// RN(q + r*y) = RN(a/d) (Markstein, IBM J. Res. Dev. 34(1), 1990; the
+// Handbook of Floating-Point Arithmetic states it for any precision p).
+// Domain: d with rcp_range(d), a == +0 or 2^-70 <= |a| <= 2^70 (a == -0
+// would come back as +0 for d > 0).
+__device__ __forceinline__ float div_rcp(float a, float d, float y) {
+    float q = a * y;
+    float r = __builtin_fmaf(-d, q, a);
+    q = __builtin_fmaf(r, y, q);
+    r = __builtin_fmaf(-d, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+
+// ---------------------------------------------------------------------------
 // Vector helpers in reference order
 // ---------------------------------------------------------------------------
 
@@ -372,6 +398,7 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
     rr.wide = 0;
     rr.sf = 0.f;
     rr.z0 = rr.z1 = rr.z2 = 0.f;
+    rr.rsf = rr.rz0 = rr.rz1 = rr.rz2 = 0.f;
 #pragma unroll
     for (int e = 0; e < 9; e++) rr.e[e] = 0;
     float W = (float)fr.W, H = (float)fr.H;
@@ -531,6 +558,17 @@ __device__ void block_tile_rect(const RasterRec* rrec, uint32_t slot0, uint32_t 
 // k_face
 // ---------------------------------------------------------------------------
 
+// Reciprocals of a record's divisors for the exact quotient path of the raster
+// stage, whose numerators are nonzero int32 edge values and barycentrics with
+// |alpha|, |beta| >= 2^-41 and gamma = +0 or |gamma| >= 2^-64. Left 0 (IEEE
+// division) for int64 triangles and divisors outside rcp_range.
+__device__ __forceinline__ void set_reciprocals(RasterRec* r) {
+    const float4 d = *reinterpret_cast<const float4*>(r);
+    if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w))
+        reinterpret_cast<float4*>(r)[1] =
+            make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
+}
+
 // Exclusive block scan of small counts (256 threads = 4 waves).
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total) {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -584,7 +622,7 @@ __device__ __forceinline__ float normal_intensity(const DevObject& o, const DevF
     return lambert(xform(o.world, q[0], q[1], q[2], w), fr.light, fr.ambient, fr.diffuse);
 }
 
-__global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs, int nobj,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_face(const DevObject* __restrict__ objs, int nobj,
                                               const DevFrame* __restrict__ frames,
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
@@ -622,77 +660,81 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
         const float4 v0 = corner(o, i0), v1 = corner(o, i1), v2 = corner(o, i2);
         if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
             st.front++;
-            float in0, in1, in2;
-            if (flags & kLighting) {
-                if (o.gouraud) {
-                    int n0 = i0, n1 = i1, n2 = i2;
-                    if (m.ni) {
-                        n0 = m.ni[3 * face + 0];
-                        n1 = m.ni[3 * face + 1];
-                        n2 = m.ni[3 * face + 2];
+            c0 = CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f};
+            c1 = CV{v1.x, v1.y, v1.z, v1.w, 0.f, 0.f, 0.f};
+            c2 = CV{v2.x, v2.y, v2.z, v2.w, 0.f, 0.f, 0.f};
+            // Plane classification needs positions only: faces the clipper
+            // rejects skip lighting and texture coordinates (which cannot
+            // change what is drawn).
+            const bool clip = (flags & kFrustumClipping) && vis != kBoxInside;
+            uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
+            if (clip) {
+                m0 = inside_mask(c0, fr.znear, fr.zfar);
+                m1 = inside_mask(c1, fr.znear, fr.zfar);
+                m2 = inside_mask(c2, fr.znear, fr.zfar);
+            }
+            const uint32_t all_in = m0 & m1 & m2;
+            // First plane not passed by every vertex (6 if none).
+            const int p = __builtin_ctz(~all_in | 0x40u);
+            // Earlier planes leave the triangle untouched; when this one
+            // rejects every vertex the pass yields an empty polygon.
+            const bool rejected = p < 6 && !(((m0 | m1 | m2) >> p) & 1u);
+            if (!rejected) {
+                float in0, in1, in2;
+                if (flags & kLighting) {
+                    if (o.gouraud) {
+                        int n0 = i0, n1 = i1, n2 = i2;
+                        if (m.ni) {
+                            n0 = m.ni[3 * face + 0];
+                            n1 = m.ni[3 * face + 1];
+                            n2 = m.ni[3 * face + 2];
+                        }
+                        in0 = normal_intensity(o, fr, n0);
+                        in1 = normal_intensity(o, fr, n1);
+                        in2 = normal_intensity(o, fr, n2);
+                    } else {
+                        const float* f = m.fn + 4 * (size_t)face;
+                        float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
+                        in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
                     }
-                    in0 = normal_intensity(o, fr, n0);
-                    in1 = normal_intensity(o, fr, n1);
-                    in2 = normal_intensity(o, fr, n2);
                 } else {
-                    const float* f = m.fn + 4 * (size_t)face;
-                    float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
-                    in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
+                    in0 = in1 = in2 = fr.ambient;
                 }
-            } else {
-                in0 = in1 = in2 = fr.ambient;
-            }
-            float u0 = 0.f, w0 = 0.f, u1 = 0.f, w1 = 0.f, u2 = 0.f, w2 = 0.f;
-            if (m.uv_vert) {
-                u0 = m.uv_vert[2 * i0];
-                w0 = m.uv_vert[2 * i0 + 1];
-                u1 = m.uv_vert[2 * i1];
-                w1 = m.uv_vert[2 * i1 + 1];
-                u2 = m.uv_vert[2 * i2];
-                w2 = m.uv_vert[2 * i2 + 1];
-            } else if (m.uv_face) {
-                const float* uv = m.uv_face + 6 * (size_t)face;
-                u0 = uv[0];
-                w0 = uv[1];
-                u1 = uv[2];
-                w1 = uv[3];
-                u2 = uv[4];
-                w2 = uv[5];
-            }
-            if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[face] : m.mesh_tex;
-            c0 = CV{v0.x, v0.y, v0.z, v0.w, u0, w0, in0};
-            c1 = CV{v1.x, v1.y, v1.z, v1.w, u1, w1, in1};
-            c2 = CV{v2.x, v2.y, v2.z, v2.w, u2, w2, in2};
-            if ((flags & kFrustumClipping) && vis != kBoxInside) {
-                const uint32_t m0 = inside_mask(c0, fr.znear, fr.zfar);
-                const uint32_t m1 = inside_mask(c1, fr.znear, fr.zfar);
-                const uint32_t m2 = inside_mask(c2, fr.znear, fr.zfar);
-                const uint32_t all_in = m0 & m1 & m2;
-                if (all_in == 0x3Fu) {
+                c0.i = in0;
+                c1.i = in1;
+                c2.i = in2;
+                if (m.uv_vert) {
+                    c0.u = m.uv_vert[2 * i0];
+                    c0.v = m.uv_vert[2 * i0 + 1];
+                    c1.u = m.uv_vert[2 * i1];
+                    c1.v = m.uv_vert[2 * i1 + 1];
+                    c2.u = m.uv_vert[2 * i2];
+                    c2.v = m.uv_vert[2 * i2 + 1];
+                } else if (m.uv_face) {
+                    const float* uv = m.uv_face + 6 * (size_t)face;
+                    c0.u = uv[0];
+                    c0.v = uv[1];
+                    c1.u = uv[2];
+                    c1.v = uv[3];
+                    c2.u = uv[4];
+                    c2.v = uv[5];
+                }
+                if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[face] : m.mesh_tex;
+                if (p == 6) {
                     // Every pass only rotates the triangle; after 6 passes it is
-                    // unchanged (clipping.go:199-232).
+                    // unchanged (clipping.go:199-232); no clipping: as is.
                     mode = 1;
                     n = 1;
                 } else {
-                    const int p = __builtin_ctz(~all_in & 0x3Fu);
-                    if (!(((m0 | m1 | m2) >> p) & 1u)) {
-                        // Earlier planes left the triangle untouched; this one
-                        // rejects every vertex (polygon2.Count == 0).
-                        n = 0;
-                    } else {
-                        nq = clip_one_plane(p, c0, c1, c2, fr.znear, fr.zfar, q);
-                        if (nq > 0) {
-                            mode = 3;
-                            n = nq - 2;
-                        } else if (nq < 0) {
-                            nq = 0;
-                            mode = 2;  // counted below, one face at a time per wave
-                        }
+                    nq = clip_one_plane(p, c0, c1, c2, fr.znear, fr.zfar, q);
+                    if (nq > 0) {
+                        mode = 3;
+                        n = nq - 2;
+                    } else if (nq < 0) {
+                        nq = 0;
+                        mode = 2;  // counted below, one face at a time per wave
                     }
                 }
-            } else {
-                mode = 1;
-                n = 1;
             }
         }
     }
@@ -771,7 +813,9 @@ __global__ __launch_bounds__(256) void k_face(const DevObject* __restrict__ objs
             for (int i = threadIdx.x; i < rw * rh; i += 256) lcnt[i] = 0u;
         __syncthreads();
         for (int i = threadIdx.x; i < total; i += 256) {
-            TileRect t = rec_tiles(fr.rrec[wg_slot0 + i], twl, thl);
+            RasterRec* rec = &fr.rrec[wg_slot0 + i];
+            set_reciprocals(rec);
+            TileRect t = rec_tiles(*rec, twl, thl);
             for (int ty = t.ty0; ty <= t.ty1; ty++)
                 for (int tx = t.tx0; tx <= t.tx1; tx++) {
                     if (in_lds)
@@ -946,61 +990,71 @@ __device__ __forceinline__ unsigned long long depth_key(float z, uint32_t seq) {
 
 constexpr unsigned long long kBackgroundKey = 0x407FFFFFull << 32;  // depth -1, seq 0
 
-struct EdgeEval {
+// Barycentrics and the reciprocal depth of one inside pixel (rasterizer.go:149-153)
+// from its edge values f12, f20: d = (sf, z0, z1, z2), rc = their reciprocals
+// or zeros (IEEE division path).
+struct Bary {
     float alpha, beta, gamma, zrec;
-    bool in;
 };
 
-// Pixel (x, y) of triangle r (rasterizer.go:147-153), evaluating the edge
-// functions directly, which equals the reference's incremental int64 walk.
-__device__ __forceinline__ EdgeEval eval_pixel(const RasterRec& r, const WideRec* wr, int x,
-                                               int y) {
-    EdgeEval e;
-    float f12f, f20f;
-    if (!r.wide) {
-        int32_t f01 = r.e[0] * x + r.e[1] * y + r.e[2];
-        int32_t f12 = r.e[3] * x + r.e[4] * y + r.e[5];
-        int32_t f20 = r.e[6] * x + r.e[7] * y + r.e[8];
-        e.in = f01 < 0 && f12 < 0 && f20 < 0;
-        f12f = (float)f12;
-        f20f = (float)f20;
+__device__ __forceinline__ Bary bary(float f12f, float f20f, const float4& d, const float4& rc) {
+    Bary b;
+    if (rc.x != 0.f) {
+        b.alpha = div_rcp(f12f, d.x, rc.x);
+        b.beta = div_rcp(f20f, d.x, rc.x);
+        b.gamma = 1.0f - b.alpha - b.beta;
+        b.zrec = -(div_rcp(b.alpha, d.y, rc.y) + div_rcp(b.beta, d.z, rc.z) +
+                   div_rcp(b.gamma, d.w, rc.w));
     } else {
-        int64_t f[3];
-        const int i0[3] = {0, 1, 2}, i1[3] = {1, 2, 0};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            int p = i0[k], q = i1[k];
-            int64_t A = wsub(wr->y[p], wr->y[q]);
-            int64_t B = wsub(wr->x[q], wr->x[p]);
-            int64_t C = wsub(wmul(wr->x[p], wr->y[q]), wmul(wr->x[q], wr->y[p]));
-            if (wr->adj >> k & 1u) C = wsub(C, 1);
-            f[k] = wadd(wadd(wmul(A, x), wmul(B, y)), C);
-        }
-        e.in = f[0] < 0 && f[1] < 0 && f[2] < 0;
-        f12f = (float)f[1];
-        f20f = (float)f[2];
+        b.alpha = f12f / d.x;
+        b.beta = f20f / d.x;
+        b.gamma = 1.0f - b.alpha - b.beta;
+        b.zrec = -(b.alpha / d.y + b.beta / d.z + b.gamma / d.w);
     }
-    e.alpha = f12f / r.sf;
-    e.beta = f20f / r.sf;
-    e.gamma = 1.0f - e.alpha - e.beta;
-    e.zrec = -(e.alpha / r.z0 + e.beta / r.z1 + e.gamma / r.z2);
-    return e;
+    return b;
 }
 
-// texture.go:69-88
+// Edge values f01, f12, f20 of pixel (x, y) of a triangle beyond the int32
+// range, in wrapping int64 arithmetic (rasterizer.go:107-129).
+__device__ __forceinline__ void wide_edges(const WideRec* wr, int x, int y, int64_t f[3]) {
+    const int i0[3] = {0, 1, 2}, i1[3] = {1, 2, 0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int p = i0[k], q = i1[k];
+        int64_t A = wsub(wr->y[p], wr->y[q]);
+        int64_t B = wsub(wr->x[q], wr->x[p]);
+        int64_t C = wsub(wmul(wr->x[p], wr->y[q]), wmul(wr->x[q], wr->y[p]));
+        if (wr->adj >> k & 1u) C = wsub(C, 1);
+        f[k] = wadd(wadd(wmul(A, x), wmul(B, y)), C);
+    }
+}
+
+// Low 32 bits of int(v) (CVTTSS2SQ).
+__device__ __forceinline__ int32_t go_int_lo(float v) {
+    if (__builtin_fabsf(v) < 2147483648.0f) return (int32_t)v;
+    return (int32_t)go_int(v);
+}
+
+// int(v) % m, Go's truncated remainder, for 0 < m < 2^31.
+__device__ __forceinline__ int32_t go_int_rem(float v, int32_t m) {
+    if (__builtin_fabsf(v) < 2147483648.0f) return (int32_t)v % m;
+    return (int32_t)(go_int(v) % (int64_t)m);
+}
+
+// texture.go:69-88. Textures hold at most 2^28 texels, so indexes fit int32.
 __device__ __forceinline__ uint32_t sample(const DevTexture& t, float u, float v) {
     switch (t.type) {
     case 0:
         return t.color;
     case 2: {
-        int64_t x = go_int((1 - u) * t.scale * t.wf) & (int64_t)(t.w - 1);
-        int64_t y = go_int(v * t.scale * t.hf) & (int64_t)(t.h - 1);
+        const int32_t x = go_int_lo((1 - u) * t.scale * t.wf) & (t.w - 1);
+        const int32_t y = go_int_lo(v * t.scale * t.hf) & (t.h - 1);
         return t.texels[y * t.w + x];
     }
     case 1: {
-        int64_t x = go_int((1 - u) * t.scale * t.wf) % (int64_t)t.w;
-        int64_t y = go_int(v * t.scale * t.hf) % (int64_t)t.h;
-        int64_t idx = y * t.w + x;
+        const int32_t x = go_int_rem((1 - u) * t.scale * t.wf, t.w);
+        const int32_t y = go_int_rem(v * t.scale * t.hf, t.h);
+        int32_t idx = y * t.w + x;
         if (idx < 0) idx = 0;
         return t.texels[idx];
     }
@@ -1017,133 +1071,200 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
     return r | (g << 8) | (b << 16) | (c & 0xFF000000u);
 }
 
+__device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Load of a 16-byte-aligned record through a global-address-space pointer
+// (global_load rather than flat, so LDS traffic is not waited on with it).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "16-byte records only");
+    typedef const __attribute__((address_space(1))) u32x4* gptr;
+    const gptr src = (gptr)p;
+    u32x4 tmp[sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); i++) tmp[i] = src[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
+
 // Exact depth of `count` queued fragments of one wave (rasterizer.go:149-156),
 // one per lane, and the (depth, sequence) max into the tile key.
 template <int TW>
-__device__ __forceinline__ void frag_depth(const uint32_t* qpix, const int32_t* qf12,
-                                           const int32_t* qf20, uint32_t head, uint32_t count,
-                                           const RasterRec* lrec, const uint32_t* lslot,
-                                           unsigned long long* key, int lane) {
+__device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t head,
+                                           uint32_t count, const RasterRec* lrec,
+                                           const uint32_t* lslot, unsigned long long* key,
+                                           int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if ((uint32_t)lane < count) {
         const uint32_t k = (head + (uint32_t)lane) & 127u;
-        const uint32_t pk = qpix[k];
-        const int32_t f12 = qf12[k], f20 = qf20[k];
+        const uint32_t pk = q[0][k];
+        const int32_t f12 = (int32_t)q[1][k], f20 = (int32_t)q[2][k];
         const uint32_t i = pk >> 12, px = (pk >> 6) & 63u, py = pk & 63u;
-        const float4 d = *reinterpret_cast<const float4*>(&lrec[i]);  // sf, z0, z1, z2
-        const float alpha = (float)f12 / d.x;
-        const float beta = (float)f20 / d.x;
-        const float gamma = 1.0f - alpha - beta;
-        const float zrec = -(alpha / d.y + beta / d.z + gamma / d.w);
-        if (zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(zrec, lslot[i] + 1u));
+        const float4* rp = reinterpret_cast<const float4*>(&lrec[i]);
+        const Bary b = bary((float)f12, (float)f20, rp[0], rp[1]);
+        if (b.zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(b.zrec, lslot[i] + 1u));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS of k_raster beyond the key array: 64 staged records and slots, the
+// per-wave fragment rings and the stats reduction.
+constexpr int kRasterChunk = 64;
+constexpr int kRasterLdsFixed =
+    kRasterChunk * (int)(sizeof(RasterRec) + 4) + 4 * 3 * 128 * 4 + 4 * 2 * 8;
+
+// Waves per SIMD the tile's LDS allows (one wave per SIMD per workgroup), at
+// most 8: the register budget follows from it (512 / waves VGPRs).
+constexpr int raster_waves(int npx) {
+    return (160 * 1024) / (npx * 8 + kRasterLdsFixed) >= 8
+               ? 8
+               : (160 * 1024) / (npx * 8 + kRasterLdsFixed);
+}
+
 template <int TW, int TH>
-__global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ frames, int nframes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
+void k_raster(const DevFrame* __restrict__ frames, int nframes,
                                                 uint32_t* __restrict__ tile_counts,
                                                 const uint32_t* __restrict__ tile_offsets,
                                                 const uint32_t* __restrict__ refs,
                                                 const DevTexture* __restrict__ textures,
                                                 unsigned long long* __restrict__ stats) {
     constexpr int NPX = TW * TH;
-    constexpr int CHUNK = 256;  // triangles staged in LDS at a time
+    constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
+    constexpr int PIECES = (int)(sizeof(RasterRec) / 16);
     __shared__ unsigned long long key[NPX];
     __shared__ RasterRec lrec[CHUNK];
     __shared__ uint32_t lslot[CHUNK];
-    __shared__ uint32_t qpix[4][128];
-    __shared__ int32_t qf12[4][128];
-    __shared__ int32_t qf20[4][128];
+    __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
     __shared__ unsigned long long red[4][2];
     const int g = blockIdx.x;
     const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
     const DevFrame& fr = frames[fi];
     const int t = g - fr.tile_begin;
     const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
+    uint32_t(*const q)[128] = qbuf[wave];
 
     for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
 
     const uint32_t cnt = tile_counts[g], off = tile_offsets[g];
-    unsigned long long covered = 0;
+    unsigned long long covered_lane = 0;  // wide path, per lane
+    uint32_t covered_wave = 0;            // int32 path, wave-uniform
     for (uint32_t base = 0; base < cnt; base += CHUNK) {
         const uint32_t n = min(cnt - base, (uint32_t)CHUNK);
-        // Stage the chunk: slots (one coalesced read), then the 64-byte raster
-        // records as four 16-byte pieces per record.
+        // Stage the chunk: slots (one coalesced read), then the raster records
+        // as 16-byte pieces.
         __syncthreads();
         if (threadIdx.x < n) lslot[threadIdx.x] = refs[off + base + threadIdx.x];
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < 4 * n; p += 256) {
-            const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[p >> 2]]);
-            reinterpret_cast<uint4*>(&lrec[p >> 2])[p & 3] = src[p & 3];
+        for (uint32_t p = threadIdx.x; p < PIECES * n; p += 256) {
+            const uint32_t ri = p / PIECES, pi = p - ri * PIECES;
+            const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[ri]]);
+            reinterpret_cast<uint4*>(&lrec[ri])[pi] = gload(src + pi);
         }
         __syncthreads();
-        // Per wave: the integer edge test runs over pixel blocks of each triangle
-        // and inside fragments are appended to an LDS ring by ballot; the exact
-        // depth (five divisions) then runs on full waves of fragments.
+        // Per wave, one triangle at a time: lanes cover an lpr x (64/lpr) pixel
+        // block of its clamped box, stepping the int32 edge values by additions
+        // (rasterizer.go:139-183 walks them the same way); inside fragments go
+        // to an LDS ring by ballot and their exact depth runs on full waves.
         uint32_t qhead = 0, qtail = 0;  // wave-uniform ring positions
         for (uint32_t i = wave; i < n; i += 4) {
-            const RasterRec r = lrec[i];
-            const int x0 = max((int)r.bx0, X0), x1 = min((int)r.bx1, X0 + TW - 1);
-            const int y0 = max((int)r.by0, Y0), y1 = min((int)r.by1, Y0 + TH - 1);
+            const RasterRec& r = lrec[i];
+            const uint32_t bxw = (uint32_t)sgpr(*reinterpret_cast<const int*>(&r.bx0));
+            const uint32_t byw = (uint32_t)sgpr(*reinterpret_cast<const int*>(&r.by0));
+            const int x0 = max((int)(int16_t)(bxw & 0xFFFFu), X0);
+            const int x1 = min((int)(int16_t)(bxw >> 16), X0 + TW - 1);
+            const int y0 = max((int)(int16_t)(byw & 0xFFFFu), Y0);
+            const int y1 = min((int)(int16_t)(byw >> 16), Y0 + TH - 1);
             const int bw = x1 - x0 + 1;
-            int lpr_log2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
-            if (lpr_log2 > 6) lpr_log2 = 6;
-            const int lpr = 1 << lpr_log2, rows = 64 >> lpr_log2;
-            const int lx = lane & (lpr - 1), ly = lane >> lpr_log2;
-            if (r.wide) {
+            int l2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
+            if (l2 > 6) l2 = 6;
+            const int lpr = 1 << l2, rows = 64 >> l2;
+            const int lx = lane & (lpr - 1), ly = lane >> l2;
+            if (sgpr((int)r.wide)) {
                 // Coordinates beyond the int32 range: direct int64 evaluation.
                 const uint32_t slot = lslot[i];
                 const WideRec* wr = &fr.wrec[slot];
+                const float4* rp = reinterpret_cast<const float4*>(&r);
                 for (int yb = y0; yb <= y1; yb += rows) {
                     const int y = yb + ly;
                     for (int xb = x0; xb <= x1; xb += lpr) {
                         const int x = xb + lx;
                         if (y > y1 || x > x1) continue;
-                        EdgeEval e = eval_pixel(r, wr, x, y);
-                        if (!e.in) continue;
-                        covered++;
-                        if (e.zrec >= -1.0f)
+                        int64_t f[3];
+                        wide_edges(wr, x, y, f);
+                        if (!(f[0] < 0 && f[1] < 0 && f[2] < 0)) continue;
+                        covered_lane++;
+                        const Bary b = bary((float)f[1], (float)f[2], rp[0], rp[1]);
+                        if (b.zrec >= -1.0f)
                             atomicMax(&key[(y - Y0) * TW + (x - X0)],
-                                      depth_key(e.zrec, slot + 1u));
+                                      depth_key(b.zrec, slot + 1u));
                     }
                 }
                 continue;
             }
+            const int A0 = sgpr(r.e[0]), B0 = sgpr(r.e[1]), C0 = sgpr(r.e[2]);
+            const int A1 = sgpr(r.e[3]), B1 = sgpr(r.e[4]), C1 = sgpr(r.e[5]);
+            const int A2 = sgpr(r.e[6]), B2 = sgpr(r.e[7]), C2 = sgpr(r.e[8]);
+            // |A|, |B| <= 2^15 and |x|, |y| < 2^14 on this path: 24-bit products
+            // are exact and every edge value fits int32.
+            const int px = x0 + lx, py = y0 + ly;
+            int f0 = __mul24(A0, px) + __mul24(B0, py) + C0;
+            int f1 = __mul24(A1, px) + __mul24(B1, py) + C1;
+            int f2 = __mul24(A2, px) + __mul24(B2, py) + C2;
+            const int sx0 = A0 * lpr, sx1 = A1 * lpr, sx2 = A2 * lpr;
+            const int sy0 = B0 * rows, sy1 = B1 * rows, sy2 = B2 * rows;
+            uint32_t prow = (i << 12) | ((uint32_t)(px - X0) << 6) | (uint32_t)(py - Y0);
             for (int yb = y0; yb <= y1; yb += rows) {
-                const int y = yb + ly;
+                // Negative exactly when the lane's row is inside the box.
+                const int gy = ly - (y1 - yb) - 1;
+                int g0 = f0, g1 = f1, g2 = f2;
+                uint32_t pk = prow;
                 for (int xb = x0; xb <= x1; xb += lpr) {
-                    const int x = xb + lx;
-                    const int32_t f01 = r.e[0] * x + r.e[1] * y + r.e[2];
-                    const int32_t f12 = r.e[3] * x + r.e[4] * y + r.e[5];
-                    const int32_t f20 = r.e[6] * x + r.e[7] * y + r.e[8];
-                    const bool in = y <= y1 && x <= x1 && f01 < 0 && f12 < 0 && f20 < 0;
-                    const uint64_t mask = __ballot(in);
-                    if (in) {
-                        const uint32_t k = (qtail + __popcll(mask & lanemask_lt(lane))) & 127u;
-                        qpix[wave][k] = (i << 12) | ((uint32_t)(x - X0) << 6) | (uint32_t)(y - Y0);
-                        qf12[wave][k] = f12;
-                        qf20[wave][k] = f20;
-                        covered++;
+                    // All five negative: the pixel is inside the box and every
+                    // edge value is < 0 (rasterizer.go:147).
+                    const int gx = lx - (x1 - xb) - 1;
+                    const int all = g0 & g1 & g2 & gx & gy;
+                    const uint64_t mask = __ballot(all < 0);
+                    if (mask) {
+                        if (all < 0) {
+                            const uint32_t k =
+                                __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo(
+                                                              (uint32_t)mask, qtail)) &
+                                127u;
+                            q[0][k] = pk;
+                            q[1][k] = (uint32_t)g1;
+                            q[2][k] = (uint32_t)g2;
+                        }
+                        const uint32_t m = (uint32_t)__popcll(mask);
+                        qtail += m;
+                        covered_wave += m;
+                        if (qtail - qhead >= 64u) {
+                            frag_depth<TW>(q, qhead, 64u, lrec, lslot, key, lane);
+                            qhead += 64u;
+                        }
                     }
-                    qtail += (uint32_t)__popcll(mask);
-                    if (qtail - qhead >= 64u) {
-                        frag_depth<TW>(qpix[wave], qf12[wave], qf20[wave], qhead, 64u, lrec,
-                                       lslot, key, lane);
-                        qhead += 64u;
-                    }
+                    g0 += sx0;
+                    g1 += sx1;
+                    g2 += sx2;
+                    pk += (uint32_t)lpr << 6;
                 }
+                f0 += sy0;
+                f1 += sy1;
+                f2 += sy2;
+                prow += (uint32_t)rows;
             }
         }
         while (qtail != qhead) {
             const uint32_t cnt_q = min(qtail - qhead, 64u);
-            frag_depth<TW>(qpix[wave], qf12[wave], qf20[wave], qhead, cnt_q, lrec, lslot, key,
-                           lane);
+            frag_depth<TW>(q, qhead, cnt_q, lrec, lslot, key, lane);
             qhead += cnt_q;
         }
     }
@@ -1178,24 +1299,34 @@ __global__ __launch_bounds__(256) void k_raster(const DevFrame* __restrict__ fra
         } else {
             shaded++;
             const uint32_t slot = seq - 1u;
-            const RasterRec r = fr.rrec[slot];
-            const ShadeRec s = fr.srec[slot];
-            const WideRec* wr = r.wide ? &fr.wrec[slot] : nullptr;
-            EdgeEval e = eval_pixel(r, wr, x, y);
+            const RasterRec r = gload(&fr.rrec[slot]);
+            const float4* rp = reinterpret_cast<const float4*>(&r);
+            const ShadeRec s = gload(&fr.srec[slot]);
+            float f12f, f20f;
+            if (!r.wide) {
+                f12f = (float)(__mul24(r.e[3], x) + __mul24(r.e[4], y) + r.e[5]);
+                f20f = (float)(__mul24(r.e[6], x) + __mul24(r.e[7], y) + r.e[8]);
+            } else {
+                int64_t f[3];
+                wide_edges(&fr.wrec[slot], x, y, f);
+                f12f = (float)f[1];
+                f20f = (float)f[2];
+            }
+            const Bary b = bary(f12f, f20f, rp[0], rp[1]);
             // rasterizer.go:158-170
-            float u = (e.alpha * s.uz[0] + e.beta * s.uz[1] + e.gamma * s.uz[2]) / e.zrec;
-            float v = (e.alpha * s.vz[0] + e.beta * s.vz[1] + e.gamma * s.vz[2]) / e.zrec;
-            float inten = e.alpha * s.in[0] + e.beta * s.in[1] + e.gamma * s.in[2];
+            float u = (b.alpha * s.uz[0] + b.beta * s.uz[1] + b.gamma * s.uz[2]) / b.zrec;
+            float v = (b.alpha * s.vz[0] + b.beta * s.vz[1] + b.gamma * s.vz[2]) / b.zrec;
+            float inten = b.alpha * s.in[0] + b.beta * s.in[1] + b.gamma * s.in[2];
             uint32_t c = s.tex < 0 ? fr.face_color : sample(textures[s.tex], u, v);
             color = color_intensity(c, inten);
-            depth = e.zrec;
+            depth = b.zrec;
         }
         const size_t idx = (size_t)y * fr.W + x;
         ((uint32_t*)fr.rgba)[idx] = color;
         fr.depth[idx] = depth;
     }
 
-    covered = wave_sum_u64(covered);
+    const unsigned long long covered = wave_sum_u64(covered_lane) + covered_wave;
     shaded = wave_sum_u64(shaded);
     if (lane == 0) {
         red[wave][0] = covered;
@@ -1259,6 +1390,107 @@ void launch_bin_fill(const LaunchParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_fill, dim3(p.face_blocks), dim3(256), 0, s, p.objs, p.nobj,
                        p.frames, p.wg_count, p.frame_overflow, p.tile_cursor, p.refs,
                        p.tw_log2, p.th_log2);
+}
+
+// Self-check of div_rcp against IEEE division: pair i is drawn from SplitMix64(seed
+// + i) in four families over the documented domain (|d| in [2^-40, 2^41),
+// |a| in [2^-70, 2^70] or +0): random floats; quotients next to a rounding
+// midpoint; divisors with extreme mantissas (all ones, powers of two); integer
+// numerators over integer divisors as the raster stage divides them.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ float make_float(uint32_t sign, int exp, uint32_t mant) {
+    return __uint_as_float((sign << 31) | ((uint32_t)(exp + 127) << 23) | (mant & 0x7FFFFFu));
+}
+
+__global__ __launch_bounds__(256) void k_check_div(uint64_t n, uint64_t seed,
+                                                   unsigned long long* mismatches,
+                                                   float* first_bad) {
+    unsigned long long bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * 256) {
+        const uint64_t r0 = splitmix64(seed + 2 * i), r1 = splitmix64(seed + 2 * i + 1);
+        const int fam = (int)(i & 3);
+        float a, d;
+        const int ed = (int)(r0 % 81) - 40;
+        if (fam == 0) {
+            d = make_float((uint32_t)(r0 >> 63), ed, (uint32_t)(r0 >> 8));
+            const int ea = (int)(r1 % 141) - 70;
+            a = ((r1 >> 40) & 255) == 0 ? 0.0f
+                                        : make_float((uint32_t)(r1 >> 63), ea, (uint32_t)(r1 >> 8));
+        } else if (fam == 1) {
+            // a = RN(d * m) with m = a float plus half an ulp: a/d lies next to m.
+            d = make_float((uint32_t)(r0 >> 63), ed, (uint32_t)(r0 >> 8));
+            const int em = (int)(r1 % 61) - 30;
+            const double m = (double)make_float(0, em, (uint32_t)(r1 >> 8)) +
+                             ldexp(1.0, em - 24) * ((r1 >> 62) & 1 ? 1.0 : -1.0);
+            a = (float)((double)d * m);
+            if (__builtin_fabsf(a) < 0x1p-70f || __builtin_fabsf(a) > 0x1p70f) a = 1.0f;
+        } else if (fam == 2) {
+            const uint32_t mant = (r0 >> 40) & 1 ? 0x7FFFFFu : ((r0 >> 41) & 1 ? 0u : 1u);
+            d = make_float((uint32_t)(r0 >> 63), ed, mant);
+            const int ea = (int)(r1 % 141) - 70;
+            a = make_float((uint32_t)(r1 >> 63), ea, (uint32_t)(r1 >> 8));
+        } else {
+            d = (float)(int32_t)(((r0 >> 16) & 0x7FFFFFFFu) | 1u) * ((r0 >> 63) ? -1.0f : 1.0f);
+            a = (float)(int32_t)(((r1 >> 16) & 0x7FFFFFFFu) | 1u) * ((r1 >> 63) ? -1.0f : 1.0f);
+            if (((r1 >> 8) & 3) == 0) a = (float)(int32_t)((r1 >> 20) & 0xFFF) + 1.0f;
+        }
+        const float y = 1.0f / d;
+        const float want = a / d;
+        const float got = div_rcp(a, d, y);
+        if (__float_as_uint(want) != __float_as_uint(got)) {
+            if (atomicAdd(mismatches, 1ull) == 0ull) {
+                first_bad[0] = a;
+                first_bad[1] = d;
+                first_bad[2] = want;
+                first_bad[3] = got;
+            }
+            bad++;
+        }
+    }
+    (void)bad;
+}
+
+void launch_check_div(uint64_t n, uint64_t seed, unsigned long long* mismatches, float* first_bad,
+                      hipStream_t s) {
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 16384);
+    if (blocks)
+        hipLaunchKernelGGL(k_check_div, dim3((unsigned)blocks), dim3(256), 0, s, n, seed,
+                           mismatches, first_bad);
+}
+
+// Plain device-to-device copy: 16-byte vector copies when both pointers and the
+// size allow it, else bytes. Also the known-size kernel that HBM counters are
+// calibrated against.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_copy8(const uint8_t* __restrict__ src,
+                                               uint8_t* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return;
+    const bool vec = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0) && bytes % 16 == 0;
+    const size_t n = vec ? bytes / 16 : bytes;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    if (vec)
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s,
+                           (const uint4*)src, (uint4*)dst, n);
+    else
+        hipLaunchKernelGGL(k_copy8, dim3((unsigned)blocks), dim3(256), 0, s,
+                           (const uint8_t*)src, (uint8_t*)dst, n);
 }
 
 template <int TW, int TH>

@@ -234,6 +234,11 @@ class DeviceBuffer:
     def fill(self, value: int) -> None:
         _lib.check(self.ctx.lib.gr_device_fill(self.ctx.handle, self.ptr, value, self.nbytes))
 
+    def copy_from(self, src: "DeviceBuffer", nbytes: int | None = None) -> None:
+        n = min(self.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= src.nbytes
+        _lib.check(self.ctx.lib.gr_device_copy(self.ctx.handle, self.ptr, src.ptr, n))
+
     def to_numpy(self, dtype, shape) -> np.ndarray:
         out = np.empty(shape, dtype)
         assert out.nbytes <= self.nbytes

@@ -164,6 +164,7 @@ gr_status gr_set_stream(gr_context *ctx, void *hip_stream);
 void *gr_get_stream(gr_context *ctx);
 
 /* ---- resident assets (HBM) ---- */
+/* Image textures hold at most 2^28 texels (GR_ERR_INVALID_ARGUMENT beyond). */
 gr_status gr_texture_create(gr_context *ctx, const gr_texture_desc *desc, const uint8_t *rgba,
                             int32_t *out_handle);
 gr_status gr_texture_destroy(gr_context *ctx, int32_t handle);
@@ -214,6 +215,9 @@ gr_status gr_device_alloc(gr_context *ctx, size_t bytes, void **out);
 gr_status gr_device_free(gr_context *ctx, void *ptr);
 /* Fill `bytes` bytes with `value` on the context stream (asynchronous). */
 gr_status gr_device_fill(gr_context *ctx, void *ptr, int32_t value, size_t bytes);
+/* Device -> device copy of `bytes` bytes on the context stream (asynchronous);
+ * the regions must not overlap. */
+gr_status gr_device_copy(gr_context *ctx, void *dst, const void *src, size_t bytes);
 /* Wait for the context stream, then copy device -> host. */
 gr_status gr_copy_to_host(gr_context *ctx, void *dst, const void *src, size_t bytes);
 
@@ -225,6 +229,16 @@ gr_status gr_set_kernel_timing(gr_context *ctx, int32_t enabled);
 int32_t gr_kernel_times(gr_context *ctx, const char **names, double *total_ms, int64_t *launches,
                         int32_t max);
 void gr_reset_kernel_times(gr_context *ctx);
+
+/* ---- self-check (no reference counterpart) ----
+ * The raster stage divides by per-triangle divisors through their correctly
+ * rounded reciprocals and two residual corrections, which equals IEEE float32
+ * division on the divisor/numerator ranges it admits. This runs `count` pairs
+ * drawn from `seed` over those ranges on the device, compares both quotients
+ * bit for bit, and returns the mismatch count; first_mismatch (4 floats, may
+ * be NULL) receives numerator, divisor, IEEE quotient, tested quotient. */
+gr_status gr_check_exact_division(gr_context *ctx, uint64_t count, uint64_t seed,
+                                  uint64_t *mismatches, float *first_mismatch);
 
 #ifdef __cplusplus
 }
