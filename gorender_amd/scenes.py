@@ -468,3 +468,79 @@ def config4_frames(n: int = 256, seed_offset: int = 4) -> List[SceneData]:
         obj = camera_object(fr, 0, (0, 0, 1.2), (0, 0, -1))
         out.append(SceneData(fr, [mesh], texs, [obj], name=f"config4-{k}"))
     return out
+
+
+@dataclass
+class ResidentSet:
+    """Meshes and textures uploaded once and drawn by many requests."""
+    meshes: List[MeshData]
+    textures: List[TextureData]
+    requests: List[SceneData]
+
+
+SERVE_RESOLUTIONS = [(640, 360), (1280, 720), (1920, 1080)]
+
+
+def config5(num_requests: int = 64, seed_offset: int = 5) -> ResidentSet:
+    """Serving form: 32 resident meshes totalling ~8 M triangles (16 displaced
+    lat-long spheres of 224k..673k triangles, 8 icospheres L6, 8 icospheres L5),
+    textures of side 2^U{4..10}; each request draws 1-8 objects (random mesh,
+    unit-quaternion rotation, translation U[-1.5, 1.5]^3, scale U[0.4, 1.2]) from
+    a camera on a shell r in [2, 6] looking at the origin, at one of three
+    resolutions. Every request lists all 32 meshes; objects refer to them by
+    index."""
+    rng = SplitMix64(SEED_BASE + seed_offset)
+    ico = {5: icosphere(5), 6: icosphere(6)}
+    meshes, textures = [], []
+    for k in range(32):
+        side = 1 << int(rng.randint(1, 4, 10)[0])
+        textures.append(random_texture(rng, side, side))
+        if k % 2 == 0:
+            F = 449_000 * (0.5 + (k // 2) / 15.0)
+            nlat = max(2, int(round(math.sqrt(F / 3.2))))
+            nlon = max(3, int(round(F / 2 / nlat)))
+            m = latlong_sphere(nlat, nlon, rng)
+        else:
+            src = ico[6 if k % 4 == 1 else 5]
+            m = MeshData(src.vertices.copy(), src.vertex_indices.copy(),
+                         src.vertex_normals.copy(), src.normal_indices.copy(), src.uvs.copy(),
+                         name=src.name)
+            p = m.vertices[:, :3].astype(np.float64)
+            u = (np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5).astype(np.float32)
+            v = (np.arccos(np.clip(p[:, 1], -1, 1)) / np.pi).astype(np.float32)
+            m.uvs = np.stack([u, v], -1)[m.vertex_indices]
+        with_texture(m, k)
+        meshes.append(m)
+    requests = []
+    for r in range(num_requests):
+        w, h = SERVE_RESOLUTIONS[int(rng.randint(1, 0, 2)[0])]
+        fr = FrameData(w, h)
+        radius = 2.0 + 4.0 * rng.uniform(1)[0]
+        z = 2.0 * rng.uniform(1)[0] - 1.0
+        phi = 2.0 * math.pi * rng.uniform(1)[0]
+        rxy = math.sqrt(max(0.0, 1.0 - z * z))
+        eye = np.array([rxy * math.cos(phi), z, rxy * math.sin(phi)]) * radius
+        direction = -eye / np.linalg.norm(eye)
+        up = (0, 1, 0) if abs(direction[1]) < 0.99 else (1, 0, 0)
+        nobj = int(rng.randint(1, 1, 8)[0])
+        objects = []
+        for _ in range(nobj):
+            mi = int(rng.randint(1, 0, 31)[0])
+            q = _shoemake(rng)
+            t = (rng.uniform(3) * 3.0 - 1.5).astype(np.float32)
+            sc = np.float32(0.4 + 0.8 * rng.uniform(1)[0])
+            objects.append(camera_object(fr, mi, tuple(eye.astype(np.float32)),
+                                         tuple(direction.astype(np.float32)), up,
+                                         world=_quat_world(q, sc, t)))
+        requests.append(SceneData(fr, meshes, textures, objects, name=f"config5-{r}"))
+    return ResidentSet(meshes, textures, requests)
+
+
+def subscene(s: SceneData) -> SceneData:
+    """The same draw with only the meshes its objects use (renumbered); textures
+    are kept whole so face texture ids stay valid."""
+    used = sorted({o.mesh for o in s.objects})
+    remap = {m: i for i, m in enumerate(used)}
+    objs = [ObjectData(remap[o.mesh], o.mvp, o.world) for o in s.objects]
+    return SceneData(s.frame, [s.meshes[m] for m in used], s.textures, objs, name=s.name)
+

@@ -1,0 +1,36 @@
+"""Writes oracle_digests.json: SHA-256 digests of the oracle's colour and depth
+buffers and its counters for every scene of scene_cases.golden_cases().
+
+Run from the repository root after building the oracle (make -C oracle):
+    python tests/golden/make_golden.py
+The digests pin the oracle and the seeded scene generators against drift; the
+GPU parity tests compare the HIP path with them as well as with the live oracle.
+"""
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import refcpu  # noqa: E402
+from parity import digest  # noqa: E402
+from scene_cases import golden_cases  # noqa: E402
+
+
+def main():
+    out = {}
+    for name, build in golden_cases():
+        scene = build()
+        rgba, depth, st = refcpu.draw(scene)
+        out[name] = digest(rgba, depth, st)
+        print(name, out[name]["triangles"], out[name]["pixels_shaded"], flush=True)
+    with open(os.path.join(HERE, "oracle_digests.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -3,6 +3,7 @@
 Each builder returns a gorender_amd.scenes.SceneData; the GPU parity tests draw
 them through the C ABI and compare with the CPU oracle.
 """
+import functools
 import math
 
 import numpy as np
@@ -182,3 +183,26 @@ def all_cases():
                                               w_range=(-1e-4, 1e-4))),
         ("degenerate", lambda: degenerate()),
     ]
+
+
+@functools.lru_cache(maxsize=1)
+def config4_batch(n=16):
+    return S.config4_frames(n)
+
+
+@functools.lru_cache(maxsize=1)
+def config5_set(n=8):
+    return S.config5(n)
+
+
+def golden_cases():
+    """Every scene with a committed oracle digest under tests/golden: the small
+    cases, configs 1-3, the first 16 frames of config 4 and the first 8 requests
+    of config 5 (as the subscene of the meshes each one draws)."""
+    cases = list(all_cases())
+    cases += [("config1-ref", lambda: S.config1("ref")), ("config1-cfg", lambda: S.config1("cfg")),
+              ("config2", S.config2), ("config3", S.config3)]
+    cases += [(f"config4-{k}", (lambda k=k: config4_batch()[k])) for k in range(16)]
+    cases += [(f"config5-{k}", (lambda k=k: S.subscene(config5_set().requests[k])))
+              for k in range(8)]
+    return cases

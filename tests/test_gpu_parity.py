@@ -11,28 +11,10 @@ from gorender_amd import _lib
 from gorender_amd import scenes as S
 from oracle import refcpu
 
+from parity import assert_same
 from scene_cases import all_cases
 
 pytestmark = pytest.mark.gpu
-
-
-def assert_same(scene, got, want, check_stats=True):
-    rgba, depth, st = got
-    rrgba, rdepth, rst = want
-    dbits, rbits = depth.view(np.uint32), rdepth.view(np.uint32)
-    bad_c = np.any(rgba != rrgba, axis=-1)
-    bad_d = dbits != rbits
-    if bad_c.any() or bad_d.any():
-        ys, xs = np.where(bad_c | bad_d)
-        y, x = int(ys[0]), int(xs[0])
-        raise AssertionError(
-            f"{scene.name}: {int(bad_c.sum())} colour / {int(bad_d.sum())} depth pixels differ; "
-            f"first at (x={x}, y={y}): gpu {rgba[y, x]} {depth[y, x]!r} "
-            f"oracle {rrgba[y, x]} {rdepth[y, x]!r}")
-    if check_stats:
-        for k in ("tpf", "triangles", "fragments_covered", "faces_front", "clip_overflow"):
-            assert st[k] == rst[k], f"{scene.name}: stat {k} gpu {st[k]} oracle {rst[k]}"
-        assert st["pixels_shaded"] == rst["pixels_written"], scene.name
 
 
 @pytest.mark.parametrize("name,builder", all_cases(), ids=[c[0] for c in all_cases()])
