@@ -40,10 +40,28 @@ def parse():
                     help="short run for rocprofv3: no CPU baseline, no latency loops")
     ap.add_argument("--mode", choices=["both", "batched", "plain"], default="both",
                     help="forms to run (rocprofv3 runs profile one form at a time)")
+    ap.add_argument("--tile", default="32x16",
+                    help="raster tile WxH, or 'auto' to time the candidates first and keep "
+                         "the fastest")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run k_copy over this many MiB (known-size kernel that the "
                          "HBM byte counters are calibrated against)")
     return ap.parse_args()
+
+
+def measured_traffic(kernel, cfg, frames_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the ratio a known-size
+    copy kernel reads under the same counters), when it was taken on this
+    kernel, tile and batch size; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_config2.txt")
+    if not os.path.exists(path):
+        return None
+    want = f"{kernel} tile={cfg.tile_w}x{cfg.tile_h} frames_per_launch={frames_per_launch}"
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch ") and want in line:
+            return int(line.rsplit("=", 1)[1])
+    return None
 
 
 def pct(xs, p):
@@ -101,6 +119,25 @@ def main():
     def run_plain(k0, nframes):
         for k in range(k0, k0 + nframes):
             ctx.submit_batch(singles[k % n_variants])
+
+    def set_tile(tw, th):
+        ctx.set_config(tile_w=tw, tile_h=th)
+
+    tile_sweep = {}
+    if args.tile == "auto":
+        # Short timed runs of the batched form per candidate; the fastest is kept.
+        for tw, th in [(16, 16), (32, 16), (16, 32), (32, 32), (64, 16)]:
+            set_tile(tw, th)
+            run_batched(0, 4 * B)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            run_batched(0, 16 * B)
+            ctx.synchronize()
+            tile_sweep[f"{tw}x{th}"] = round(16 * B / (time.perf_counter() - t0), 1)
+        best = max(tile_sweep, key=tile_sweep.get)
+        set_tile(*[int(v) for v in best.split("x")])
+    else:
+        set_tile(*[int(v) for v in args.tile.split("x")])
 
     do_b = args.mode in ("both", "batched")
     do_p = args.mode in ("both", "plain")
@@ -194,25 +231,35 @@ def main():
     dom_bytes = alg_bytes.get(dom_name, 0) * frames_per_launch
     achieved = dom_bytes / avg_launch_s / 1e9 if dom_bytes else 0.0
 
-    cpu = None
+    cpu, cpu_forms = None, None
     if not args.no_cpu and not args.profile:
         from oracle import refcpu
+        ps = refcpu.PreparedScene(variants[0])
+
+        def time_cpu(threads, tiles, seconds):
+            refcpu.draw(variants[0], ps, parallel=threads, tiles=(tiles, tiles))
+            t0 = time.perf_counter()
+            n = 0
+            while time.perf_counter() - t0 < seconds or n < 3:
+                refcpu.draw(variants[0], ps, parallel=threads, tiles=(tiles, tiles))
+                n += 1
+            dt = time.perf_counter() - t0
+            return faces * n / dt / 1e6, n, dt
+
+        # par16: the reference's parallel structure (object-parallel geometry,
+        # 4x4 tile raster, 16 workers); seq: one thread, sequential draw order.
         threads = min(16, os.cpu_count() or 1)
         tiles = int(math.ceil(math.sqrt(threads)))
-        ps = refcpu.PreparedScene(variants[0])
-        refcpu.draw(variants[0], ps, parallel=threads, tiles=(tiles, tiles))
-        t0 = time.perf_counter()
-        n = 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            refcpu.draw(variants[0], ps, parallel=threads, tiles=(tiles, tiles))
-            n += 1
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(faces * n / dt / 1e6, 3), "unit": "Mtris/s", "cores": threads,
+        v16, n16, dt16 = time_cpu(threads, tiles, 0.7 * args.cpu_seconds)
+        v1, n1, dt1 = time_cpu(0, 1, 0.3 * args.cpu_seconds)
+        cpu = {"value": round(v16, 3), "unit": "Mtris/s", "cores": threads,
                "kind": "port",
-               "sample": f"{n} frames of the same 200k-triangle 1280x720 frame, "
+               "sample": f"{n16} frames of the same 200k-triangle 1280x720 frame, "
                          f"oracle/refcpu.c rc_draw_parallel ({threads} threads, "
-                         f"{tiles}x{tiles} tiles), {dt:.1f} s; a C restatement, not "
+                         f"{tiles}x{tiles} tiles), {dt16:.1f} s; a C restatement, not "
                          f"gorender's Go binary"}
+        cpu_forms = {"par16": round(v16, 3), "seq": round(v1, 3), "seq_frames": n1,
+                     "unit": "Mtris/s", "host_cpus": os.cpu_count()}
 
     out = {
         "metric": "Mtris/s + shaded Mpix/s @1280x720; p50/p99 frame latency, batched vs plain",
@@ -250,10 +297,13 @@ def main():
                                           for k, v in kt_plain.items()}},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(dom_name, ctx.config(), frames_per_launch),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "algorithmic_bytes_per_launch": dom_bytes},
         "cpu_baseline": cpu,
+        "cpu_baseline_forms": cpu_forms,
+        "tile_sweep_fps": tile_sweep or None,
     }
     print(json.dumps(out))
     ctx.close()

@@ -119,6 +119,19 @@ SIGNATURES = {
     "gr_device_free": (c_int32, [c_void_p, c_void_p]),
     "gr_device_fill": (c_int32, [c_void_p, c_void_p, c_int32, ctypes.c_size_t]),
     "gr_device_copy": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
+    "gr_obj_open": (c_int32, [ctypes.c_char_p, c_int32, POINTER(c_void_p)]),
+    "gr_obj_close": (None, [c_void_p]),
+    "gr_obj_num_meshes": (c_int32, [c_void_p]),
+    "gr_obj_num_textures": (c_int32, [c_void_p]),
+    "gr_obj_mesh": (c_int32, [c_void_p, c_int32, POINTER(gr_mesh_desc)]),
+    "gr_obj_texture": (c_int32, [c_void_p, c_int32, POINTER(gr_texture_desc),
+                                 POINTER(POINTER(ctypes.c_uint8))]),
+    "gr_obj_upload": (c_int32, [c_void_p, c_void_p, POINTER(c_int32), c_int32,
+                                POINTER(c_int32)]),
+    "gr_png_decode": (c_int32, [ctypes.c_char_p, POINTER(c_int32), POINTER(c_int32),
+                                POINTER(POINTER(ctypes.c_uint8))]),
+    "gr_host_free": (None, [c_void_p]),
+    "gr_texture_load": (c_int32, [c_void_p, ctypes.c_char_p, POINTER(c_int32)]),
     "gr_check_exact_division": (c_int32, [c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                           POINTER(ctypes.c_uint64), POINTER(ctypes.c_float)]),
     "gr_copy_to_host": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),

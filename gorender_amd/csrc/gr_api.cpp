@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "gorender_hip.h"
+#include "gr_error.h"
 #include "gr_device.h"
 #include "gr_launch.h"
 
@@ -67,6 +68,10 @@ struct TimedLaunch {
 };
 
 }  // namespace
+
+namespace gr_internal {
+gr_status fail(gr_status s, const std::string& msg) { return ::fail(s, msg); }
+}  // namespace gr_internal
 
 struct gr_context {
     int device = 0;
@@ -356,7 +361,13 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     const size_t off_stats = off_objs + align256((size_t)nobj * sizeof(gr::DevObject));
     const size_t off_counters = off_stats + align256((size_t)nf * gr::kNumStats * 8);
     const size_t off_over = off_counters + align256(sizeof(gr::DevBatchCounters));
-    const size_t batch_bytes = off_over + align256((size_t)nf * 4);
+    const size_t off_xcd = off_over + align256((size_t)nf * 4);
+    const size_t xcd_ints = 8 * ((size_t)nf + 1);
+    const size_t batch_bytes = off_xcd + align256(xcd_ints * sizeof(int));
+    std::vector<int> xcd(xcd_ints), nface(nf);
+    for (int f = 0; f < nf; f++) nface[f] = frames[f].face_blocks;
+    int face_grid = 0;
+    gr::xcd_tables(nface.data(), nf, xcd.data(), &face_grid);
 
     gr_status st;
     if ((st = ensure(c, c->batch, batch_bytes))) return st;
@@ -392,6 +403,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         std::memset(h, 0, batch_bytes);
         std::memcpy(h, frames.data(), (size_t)nf * sizeof(gr::DevFrame));
         if (nobj) std::memcpy((char*)h + off_objs, dobj.data(), (size_t)nobj * sizeof(gr::DevObject));
+        std::memcpy((char*)h + off_xcd, xcd.data(), xcd.size() * sizeof(int));
         GR_HIP(hipMemcpyAsync(c->batch.p, h, batch_bytes, hipMemcpyHostToDevice, c->stream));
         if ((st = staging_mark(c, slot))) return st;
     }
@@ -417,6 +429,8 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     p.stats = (unsigned long long*)(bb + off_stats);
     p.counters = (gr::DevBatchCounters*)(bb + off_counters);
     p.ctrl = (gr::DevControl*)c->ctrl.p;
+    p.xcd_face = (const int*)(bb + off_xcd);
+    p.face_grid = face_grid;
 
     timed_launch(c, "k_face", [&] { gr::launch_face(p, c->stream); });
     timed_launch(c, "k_bin_scan", [&] { gr::launch_bin_scan(p, c->stream); });

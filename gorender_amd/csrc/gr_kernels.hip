@@ -185,6 +185,29 @@ __device__ __forceinline__ bool facing_camera(const float4& v0, const float4& v1
     return d > 0.f;
 }
 
+// Units x, x + 8, x + 16, ... of a frame with n units run on XCD x.
+__host__ __device__ __forceinline__ int xcd_count(int x, int n) {
+    return n > x ? (n - x + 7) >> 3 : 0;
+}
+
+template <typename F>
+__device__ __forceinline__ int last_le(int n, int b, F key);
+
+// Workgroup h -> (frame, unit within the frame) in the XCD-aware order: the
+// hardware places workgroup h on XCD h % 8, and XCD x takes units x, x + 8, ...
+// of every frame, frame after frame, so a unit's inputs stay in one XCD's L2
+// across the frames of a batch while neighbouring units (whose cost varies
+// together) spread over all XCDs. Returns false for the padding workgroups.
+__device__ __forceinline__ bool xcd_unit(const int* table, int nframes, int h, int& f,
+                                         int& local) {
+    const int x = h & 7, k = h >> 3;
+    const int* P = table + x * (nframes + 1);
+    if (k >= P[nframes]) return false;
+    f = last_le(nframes, k, [&](int i) { return P[i]; });
+    local = x + 8 * (k - P[f]);
+    return true;
+}
+
 // Binary search: last index i in [0, n) with key(i) <= b.
 template <typename F>
 __device__ __forceinline__ int last_le(int n, int b, F key) {
@@ -627,14 +650,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
                                               unsigned long long* __restrict__ stats, int twl,
-                                              int thl) {
+                                              int thl, const int* __restrict__ xcd_face,
+                                              int nframes) {
     __shared__ int lds_scan[4];
     __shared__ unsigned long long lds_st[4][4];
     __shared__ CV wpoly[4][2][kMaxClip];
     __shared__ uint32_t lcnt[kLdsTiles];
     __shared__ int lrect[4];
 
-    const int b = blockIdx.x;
+    int bf, blocal;
+    if (!xcd_unit(xcd_face, nframes, (int)blockIdx.x, bf, blocal)) return;
+    const int b = frames[bf].face_block_begin + blocal;
     const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
     const DevObject& o = objs[oi];
     const DevMesh& m = o.m;
@@ -1375,8 +1401,21 @@ void launch_face_normals(const float* pos, const int32_t* vi, int F, float* fn, 
 
 void launch_face(const LaunchParams& p, hipStream_t s) {
     if (p.face_blocks == 0) return;
-    hipLaunchKernelGGL(k_face, dim3(p.face_blocks), dim3(256), 0, s, p.objs, p.nobj, p.frames,
-                       p.tile_counts, p.wg_count, p.stats, p.tw_log2, p.th_log2);
+    hipLaunchKernelGGL(k_face, dim3(p.face_grid), dim3(256), 0, s, p.objs, p.nobj, p.frames,
+                       p.tile_counts, p.wg_count, p.stats, p.tw_log2, p.th_log2, p.xcd_face,
+                       p.nframes);
+}
+
+void xcd_tables(const int* n, int nframes, int* table, int* grid) {
+    int longest = 0;
+    for (int x = 0; x < 8; x++) {
+        int* P = table + x * (nframes + 1);
+        P[0] = 0;
+        for (int f = 0; f < nframes; f++)
+            P[f + 1] = P[f] + xcd_count(x, n[f]);
+        longest = std::max(longest, P[nframes]);
+    }
+    *grid = 8 * longest;
 }
 
 void launch_bin_scan(const LaunchParams& p, hipStream_t s) {

@@ -29,7 +29,15 @@ struct LaunchParams {
     unsigned long long* stats;  // [nframes * kNumStats]
     DevBatchCounters* counters;
     DevControl* ctrl;
+    // XCD-aware face-block order: xcd_face[x * (nframes + 1) + f] = face blocks
+    // of frames < f that XCD x runs (blocks x, x + 8, ... of every frame); the
+    // grid is 8 * the longest XCD list.
+    const int* xcd_face;
+    int face_grid;
 };
+
+// Prefix tables of the XCD-aware order for per-frame unit counts `n`.
+void xcd_tables(const int* n, int nframes, int* table, int* grid);
 
 void launch_face_normals(const float* pos, const int32_t* vi, int F, float* fn, hipStream_t s);
 void launch_face(const LaunchParams& p, hipStream_t s);

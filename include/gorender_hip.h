@@ -221,6 +221,39 @@ gr_status gr_device_copy(gr_context *ctx, void *dst, const void *src, size_t byt
 /* Wait for the context stream, then copy device -> host. */
 gr_status gr_copy_to_host(gr_context *ctx, void *dst, const void *src, size_t bytes);
 
+/* ---- asset ingestion (host side; no device needed to parse) ---- */
+typedef struct gr_obj_file gr_obj_file;
+/* LoadObjFile (obj.go:196-309): parse an OBJ file, the MTL libraries it names
+ * (parseMtlLibFile, obj.go:156-193) and their PNG map_Kd images into host
+ * memory, one mesh per "o" group unless single_mesh. Line, face and number
+ * rules are the reference's (fmt.Sscanf formats of obj.go:42-151, including the
+ * "f v//vn" normal-index bug of obj.go:77-82). Malformed input ->
+ * GR_ERR_INVALID_ARGUMENT; indexes the reference would panic on while parsing
+ * -> GR_ERR_INDEX_OUT_OF_RANGE. */
+gr_status gr_obj_open(const char *path, int32_t single_mesh, gr_obj_file **out);
+void gr_obj_close(gr_obj_file *obj);
+int32_t gr_obj_num_meshes(const gr_obj_file *obj);
+int32_t gr_obj_num_textures(const gr_obj_file *obj);
+/* Arrays of mesh i, valid until gr_obj_close; face_textures index the file's
+ * texture table (gr_obj_texture), -1 = nil. Face normals are left to
+ * gr_mesh_create (NewMesh, mesh.go:53-69). */
+gr_status gr_obj_mesh(const gr_obj_file *obj, int32_t i, gr_mesh_desc *desc);
+/* Texture i of the file: a decoded map_Kd image, or the solid magenta default
+ * the reference gives materials without one (obj.go:219,231-234). */
+gr_status gr_obj_texture(const gr_obj_file *obj, int32_t i, gr_texture_desc *desc,
+                         const uint8_t **rgba);
+/* Upload every texture and mesh of the file to the resident store; one mesh
+ * handle per mesh (GR_ERR_CAPACITY, uploading nothing, if max_meshes is short). */
+gr_status gr_obj_upload(gr_context *ctx, const gr_obj_file *obj, int32_t *mesh_handles,
+                        int32_t max_meshes, int32_t *num_meshes);
+/* image/png decoding as LoadTextureFile uses it (texture.go:91-103), pixels
+ * converted by color.RGBAModel (texture.go:57: alpha premultiplied); *rgba is
+ * allocated here, release it with gr_host_free. JPEG -> GR_ERR_UNSUPPORTED. */
+gr_status gr_png_decode(const char *path, int32_t *width, int32_t *height, uint8_t **rgba);
+void gr_host_free(void *ptr);
+/* LoadTextureFile + gr_texture_create (type from gr_texture_kind_for_size). */
+gr_status gr_texture_load(gr_context *ctx, const char *path, int32_t *out_handle);
+
 /* ---- measurement ---- */
 /* When enabled, each kernel launch is bracketed by HIP events on the context
  * stream and its duration accumulated per kernel name. */
