@@ -40,7 +40,7 @@ def parse():
                     help="short run for rocprofv3: no CPU baseline, no latency loops")
     ap.add_argument("--mode", choices=["both", "batched", "plain"], default="both",
                     help="forms to run (rocprofv3 runs profile one form at a time)")
-    ap.add_argument("--tile", default="32x16",
+    ap.add_argument("--tile", default="32x32",
                     help="raster tile WxH, or 'auto' to time the candidates first and keep "
                          "the fastest")
     ap.add_argument("--calib-copy-mib", type=int, default=0,

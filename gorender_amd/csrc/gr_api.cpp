@@ -91,7 +91,7 @@ struct gr_context {
     int32_t last_frames = 0;
     const unsigned long long* last_stats = nullptr;  // device stats of the last draw
 
-    gr_config cfg{32, 16, 1 << 22, 4, 0};
+    gr_config cfg{32, 32, 1 << 22, 4, 0};
 
     bool timing = false;
     std::vector<TimedLaunch> timed;

@@ -392,14 +392,25 @@ struct FaceStats {
 // then the per-triangle part of rasterizer.go:98-137). Returns the TPF
 // references in the low 16 bits and 1 << 16 when the triangle was binned.
 __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, const CV& c,
-                                  int tex, uint32_t slot) {
+                                  int tex, uint32_t slot, uint32_t* tile_counts, int twl,
+                                  int thl) {
     uint32_t tpf = 0, tris = 0;
     const CV vv[3] = {a, b, c};
     float sx[3], sy[3], zw[3];
+    // The screen x and y rows use z/w only through screen[2] and screen[6]: when
+    // both are zero, z/w matters only by being infinite or NaN (0 * z/w), and the
+    // sign of a zero term never changes what the rasteriser does with sx, sy
+    // (int(), comparisons). w/w is 1 for finite nonzero w, else NaN.
+    const bool zcol = fr.screen[2] != 0.f || fr.screen[6] != 0.f;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        float w = vv[j].w;
-        float px = vv[j].x / w, py = vv[j].y / w, pz = vv[j].z / w, pw = w / w;
+        const float w = vv[j].w, z = vv[j].z;
+        const float px = vv[j].x / w, py = vv[j].y / w;
+        const bool wok = __builtin_isfinite(w) && w != 0.f;
+        const float pw = wok ? 1.0f : __builtin_nanf("");
+        float pz = 0.0f;
+        if (zcol || !(__builtin_isfinite(z) && __builtin_fabsf(z) < __builtin_fabsf(w) * 0x1p100f))
+            pz = z / w;
         sx[j] = fr.screen[0] * px + fr.screen[1] * py + fr.screen[2] * pz + fr.screen[3] * pw;
         sy[j] = fr.screen[4] * px + fr.screen[5] * py + fr.screen[6] * pz + fr.screen[7] * pw;
         zw[j] = w;
@@ -408,10 +419,20 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
     float fmaxx = go_max(go_max(sx[0], sx[1]), sx[2]);
     float fminy = go_min(go_min(sy[0], sy[1]), sy[2]);
     float fmaxy = go_max(go_max(sy[0], sy[1]), sy[2]);
-    // renderer.go:226-244 against the reference TPF grid.
-    for (int t = 0; t < fr.ref_tiles; t++) {
-        const float* r = fr.ref_tile_rect[t];
-        if (fmaxx >= r[0] && fminx <= r[2] && fmaxy >= r[1] && fminy <= r[3]) tpf++;
+    // renderer.go:226-244 against the reference TPF grid. Its tiles are a
+    // product grid (x bounds from tile % n, y bounds from tile / n,
+    // renderer.go:61-67), so the overlapping tiles are the overlapping columns
+    // times the overlapping rows.
+    {
+        const int ng = fr.ref_tiles == 16 ? 4 : 1;
+        uint32_t cols = 0, rows = 0;
+        for (int c = 0; c < ng; c++) {
+            const float* rx = fr.ref_tile_rect[c];
+            const float* ry = fr.ref_tile_rect[c * ng];
+            cols += (uint32_t)(fmaxx >= rx[0] && fminx <= rx[2]);
+            rows += (uint32_t)(fmaxy >= ry[1] && fminy <= ry[3]);
+        }
+        tpf = cols * rows;
     }
     RasterRec rr;
     rr.bx0 = 1;
@@ -453,38 +474,48 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                 fast = fast && ix[j] >= -kFastCoord && ix[j] <= kFastCoord &&
                        iy[j] >= -kFastCoord && iy[j] <= kFastCoord;
             // Edge functions f_e(x,y) = A*x + B*y + C (rasterizer.go:107-117) with the
-            // top-left adjustment folded into C (rasterizer.go:120-129).
+            // top-left adjustment folded into C (rasterizer.go:120-129). The pixel
+            // sum fx12 + fx20 + fx01 is the sum of the C's (the A and B terms
+            // cancel).
             const int i0[3] = {0, 1, 2}, i1[3] = {1, 2, 0};
-            int64_t csum = 0;
-            uint32_t adj = 0;
-            int64_t A64[3], B64[3], C64[3];
-#pragma unroll
-            for (int e = 0; e < 3; e++) {
-                int p = i0[e], q = i1[e];
-                int64_t A = wsub(iy[p], iy[q]);
-                int64_t B = wsub(ix[q], ix[p]);
-                int64_t C = wsub(wmul(ix[p], iy[q]), wmul(ix[q], iy[p]));
-                bool keep = B > 0 || (B == 0 && A > 0);
-                if (!keep) {
-                    C = wsub(C, 1);
-                    adj |= 1u << e;
-                }
-                A64[e] = A;
-                B64[e] = B;
-                C64[e] = C;
-                csum = wadd(csum, C);
-            }
-            // The pixel sum fx12 + fx20 + fx01 is this constant (the A and B
-            // terms cancel); wrapping addition makes the order irrelevant.
-            rr.sf = (float)csum;
             if (fast) {
+                // |x|, |y| <= 2^14: 24-bit products, and every value (the sum of
+                // the three C's included) fits int32.
+                int32_t X[3], Y[3], csum = 0;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    X[j] = (int32_t)ix[j];
+                    Y[j] = (int32_t)iy[j];
+                }
 #pragma unroll
                 for (int e = 0; e < 3; e++) {
-                    rr.e[3 * e + 0] = (int32_t)A64[e];
-                    rr.e[3 * e + 1] = (int32_t)B64[e];
-                    rr.e[3 * e + 2] = (int32_t)C64[e];
+                    const int p = i0[e], q = i1[e];
+                    const int32_t A = Y[p] - Y[q], B = X[q] - X[p];
+                    int32_t C = __mul24(X[p], Y[q]) - __mul24(X[q], Y[p]);
+                    if (!(B > 0 || (B == 0 && A > 0))) C -= 1;
+                    rr.e[3 * e + 0] = A;
+                    rr.e[3 * e + 1] = B;
+                    rr.e[3 * e + 2] = C;
+                    csum += C;
                 }
+                rr.sf = (float)csum;
             } else {
+                // Wrapping int64 arithmetic as the reference's int.
+                int64_t csum = 0;
+                uint32_t adj = 0;
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    const int p = i0[e], q = i1[e];
+                    const int64_t A = wsub(iy[p], iy[q]);
+                    const int64_t B = wsub(ix[q], ix[p]);
+                    int64_t C = wsub(wmul(ix[p], iy[q]), wmul(ix[q], iy[p]));
+                    if (!(B > 0 || (B == 0 && A > 0))) {
+                        C = wsub(C, 1);
+                        adj |= 1u << e;
+                    }
+                    csum = wadd(csum, C);
+                }
+                rr.sf = (float)csum;
                 rr.wide = 1;
                 WideRec wr;
 #pragma unroll
@@ -497,13 +528,13 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                 fr.wrec[slot] = wr;
             }
             ShadeRec sr;
-            // rasterizer.go:132-137
-            sr.uz[0] = a.u / zw[0];
-            sr.vz[0] = a.v / zw[0];
-            sr.uz[1] = b.u / zw[1];
-            sr.vz[1] = b.v / zw[1];
-            sr.uz[2] = c.u / zw[2];
-            sr.vz[2] = c.v / zw[2];
+            // u, v as clipped; finalize_record divides them by z (rasterizer.go:132-137).
+            sr.uz[0] = a.u;
+            sr.vz[0] = a.v;
+            sr.uz[1] = b.u;
+            sr.vz[1] = b.v;
+            sr.uz[2] = c.u;
+            sr.vz[2] = c.v;
             sr.in[0] = a.i;
             sr.in[1] = b.i;
             sr.in[2] = c.i;
@@ -513,8 +544,34 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
         }
     }
     fr.rrec[slot] = rr;
+    // Tile references of the triangle, counted for the list scan.
+    if (rr.bx0 <= rr.bx1) {
+        const int tx0 = (int)rr.bx0 >> twl, tx1 = (int)rr.bx1 >> twl;
+        const int ty0 = (int)rr.by0 >> thl, ty1 = (int)rr.by1 >> thl;
+        for (int ty = ty0; ty <= ty1; ty++)
+            for (int tx = tx0; tx <= tx1; tx++)
+                atomicAdd(&tile_counts[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
+    }
     return tpf | (tris << 16);
 }
+
+// A slot that holds no triangle: an empty box, never binned or finalized.
+__device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t slot) {
+    RasterRec rr;
+    rr.sf = rr.z0 = rr.z1 = rr.z2 = 0.f;
+    rr.rsf = rr.rz0 = rr.rz1 = rr.rz2 = 0.f;
+    rr.bx0 = 1;
+    rr.bx1 = 0;
+    rr.by0 = 1;
+    rr.by1 = 0;
+    rr.wide = 0;
+#pragma unroll
+    for (int e = 0; e < 9; e++) rr.e[e] = 0;
+    fr.rrec[slot] = rr;
+}
+
+// General faces clipped per round by one wave; each fills kMaxFan lanes.
+constexpr int kStage = 64 / kMaxFan;
 
 __device__ __forceinline__ void add_emit(FaceStats& st, uint32_t e) {
     st.tpf += e & 0xFFFFu;
@@ -581,12 +638,21 @@ __device__ void block_tile_rect(const RasterRec* rrec, uint32_t slot0, uint32_t 
 // k_face
 // ---------------------------------------------------------------------------
 
-// Reciprocals of a record's divisors for the exact quotient path of the raster
-// stage, whose numerators are nonzero int32 edge values and barycentrics with
-// |alpha|, |beta| >= 2^-41 and gamma = +0 or |gamma| >= 2^-64. Left 0 (IEEE
-// division) for int64 triangles and divisors outside rcp_range.
-__device__ __forceinline__ void set_reciprocals(RasterRec* r) {
+// Per-triangle divisions only the raster stage needs, done once per record
+// after the face stage: u/z and v/z (rasterizer.go:132-137; the face stage
+// stores u and v) and the reciprocals of the divisors for the exact quotient
+// path, whose numerators are nonzero int32 edge values and barycentrics with
+// |alpha|, |beta| >= 2^-41 and gamma = +0 or |gamma| >= 2^-64 (left 0, i.e.
+// IEEE division, for int64 triangles and divisors outside rcp_range).
+__device__ __forceinline__ void finalize_record(RasterRec* r, ShadeRec* sr) {
     const float4 d = *reinterpret_cast<const float4*>(r);
+    if (r->bx0 > r->bx1) return;  // not rasterised; no shading inputs were written
+    sr->uz[0] = sr->uz[0] / d.y;
+    sr->vz[0] = sr->vz[0] / d.y;
+    sr->uz[1] = sr->uz[1] / d.z;
+    sr->vz[1] = sr->vz[1] / d.z;
+    sr->uz[2] = sr->uz[2] / d.w;
+    sr->vz[2] = sr->vz[2] / d.w;
     if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w))
         reinterpret_cast<float4*>(r)[1] =
             make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
@@ -654,9 +720,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                               int nframes) {
     __shared__ int lds_scan[4];
     __shared__ unsigned long long lds_st[4][4];
+    __shared__ int lds_nsurv[4];                     // survivors per wave
+    __shared__ uint32_t lds_sface[kFacesPerWg];      // survivor: face offset | plane << 8
+    __shared__ float lds_spos[kFacesPerWg][12];      // survivor: clip-space corners
     __shared__ CV wpoly[4][2][kMaxClip];
-    __shared__ uint32_t lcnt[kLdsTiles];
-    __shared__ int lrect[4];
+    __shared__ CV lds_stage[4][kStage][kMaxClip];  // clipped polygons of general faces
+    __shared__ int lds_stage_k[4][kStage], lds_stage_tex[4][kStage], lds_stage_slot[4][kStage];
 
     int bf, blocal;
     if (!xcd_unit(xcd_face, nframes, (int)blockIdx.x, bf, blocal)) return;
@@ -672,6 +741,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int vis = box_visibility(o, fr, lane);
 
     FaceStats st = {0, 0, 0, 0};
+
+    // Pass 1, one face per thread: clip-space corners, backface test and plane
+    // classification. The survivors (front faces the clipper does not reject
+    // outright) are compacted in face order into LDS with their corners, so the
+    // lighting, clipping and set-up below run on dense waves.
+    bool survivor = false;
+    int p = 6;
+    float4 v0, v1, v2;
+    if (face < m.F && vis != kBoxOutside) {
+        const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
+        v0 = corner(o, i0);
+        v1 = corner(o, i1);
+        v2 = corner(o, i2);
+        if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
+            st.front++;
+            uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
+            if ((flags & kFrustumClipping) && vis != kBoxInside) {
+                m0 = inside_mask(CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+                m1 = inside_mask(CV{v1.x, v1.y, v1.z, v1.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+                m2 = inside_mask(CV{v2.x, v2.y, v2.z, v2.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+            }
+            const uint32_t all_in = m0 & m1 & m2;
+            // First plane not passed by every vertex (6 if none). Earlier planes
+            // leave the triangle untouched; when this one rejects every vertex
+            // the pass yields an empty polygon.
+            p = __builtin_ctz(~all_in | 0x40u);
+            survivor = !(p < 6 && !(((m0 | m1 | m2) >> p) & 1u));
+        }
+    }
+    {
+        const uint64_t sm = __ballot(survivor);
+        if (lane == 0) lds_nsurv[wave] = __popcll(sm);
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wave; w++) base += lds_nsurv[w];
+        if (survivor) {
+            const int si = base + __popcll(sm & lanemask_lt(lane));
+            lds_sface[si] = (uint32_t)threadIdx.x | ((uint32_t)p << 8);
+            float4* dst = reinterpret_cast<float4*>(lds_spos[si]);
+            dst[0] = v0;
+            dst[1] = v1;
+            dst[2] = v2;
+        }
+        __syncthreads();
+    }
+    const int nsurv = lds_nsurv[0] + lds_nsurv[1] + lds_nsurv[2] + lds_nsurv[3];
+
+    // Pass 2, survivor s on thread s: lighting, texture coordinates, clipping.
     // 0: nothing, 1: emit the triangle as is, 2: full clipper in the wave,
     // 3: one-plane polygon in q[0..nq-1]
     int mode = 0;
@@ -680,131 +797,128 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     CV q[4];
     int nq = 0;
     int tex = -1;
-
-    if (face < m.F && vis != kBoxOutside) {
-        const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
-        const float4 v0 = corner(o, i0), v1 = corner(o, i1), v2 = corner(o, i2);
-        if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
-            st.front++;
-            c0 = CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f};
-            c1 = CV{v1.x, v1.y, v1.z, v1.w, 0.f, 0.f, 0.f};
-            c2 = CV{v2.x, v2.y, v2.z, v2.w, 0.f, 0.f, 0.f};
-            // Plane classification needs positions only: faces the clipper
-            // rejects skip lighting and texture coordinates (which cannot
-            // change what is drawn).
-            const bool clip = (flags & kFrustumClipping) && vis != kBoxInside;
-            uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
-            if (clip) {
-                m0 = inside_mask(c0, fr.znear, fr.zfar);
-                m1 = inside_mask(c1, fr.znear, fr.zfar);
-                m2 = inside_mask(c2, fr.znear, fr.zfar);
-            }
-            const uint32_t all_in = m0 & m1 & m2;
-            // First plane not passed by every vertex (6 if none).
-            const int p = __builtin_ctz(~all_in | 0x40u);
-            // Earlier planes leave the triangle untouched; when this one
-            // rejects every vertex the pass yields an empty polygon.
-            const bool rejected = p < 6 && !(((m0 | m1 | m2) >> p) & 1u);
-            if (!rejected) {
-                float in0, in1, in2;
-                if (flags & kLighting) {
-                    if (o.gouraud) {
-                        int n0 = i0, n1 = i1, n2 = i2;
-                        if (m.ni) {
-                            n0 = m.ni[3 * face + 0];
-                            n1 = m.ni[3 * face + 1];
-                            n2 = m.ni[3 * face + 2];
-                        }
-                        in0 = normal_intensity(o, fr, n0);
-                        in1 = normal_intensity(o, fr, n1);
-                        in2 = normal_intensity(o, fr, n2);
-                    } else {
-                        const float* f = m.fn + 4 * (size_t)face;
-                        float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
-                        in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
-                    }
-                } else {
-                    in0 = in1 = in2 = fr.ambient;
+    if ((int)threadIdx.x < nsurv) {
+        const uint32_t e = lds_sface[threadIdx.x];
+        const int sf = wg_in_obj * kFacesPerWg + (int)(e & 0xFFu);
+        const int sp = (int)(e >> 8);
+        const float4* src = reinterpret_cast<const float4*>(lds_spos[threadIdx.x]);
+        const float4 w0 = src[0], w1 = src[1], w2 = src[2];
+        c0 = CV{w0.x, w0.y, w0.z, w0.w, 0.f, 0.f, 0.f};
+        c1 = CV{w1.x, w1.y, w1.z, w1.w, 0.f, 0.f, 0.f};
+        c2 = CV{w2.x, w2.y, w2.z, w2.w, 0.f, 0.f, 0.f};
+        const int i0 = m.vi[3 * sf + 0], i1 = m.vi[3 * sf + 1], i2 = m.vi[3 * sf + 2];
+        float in0, in1, in2;
+        if (flags & kLighting) {
+            if (o.gouraud) {
+                int n0 = i0, n1 = i1, n2 = i2;
+                if (m.ni) {
+                    n0 = m.ni[3 * sf + 0];
+                    n1 = m.ni[3 * sf + 1];
+                    n2 = m.ni[3 * sf + 2];
                 }
-                c0.i = in0;
-                c1.i = in1;
-                c2.i = in2;
-                if (m.uv_vert) {
-                    c0.u = m.uv_vert[2 * i0];
-                    c0.v = m.uv_vert[2 * i0 + 1];
-                    c1.u = m.uv_vert[2 * i1];
-                    c1.v = m.uv_vert[2 * i1 + 1];
-                    c2.u = m.uv_vert[2 * i2];
-                    c2.v = m.uv_vert[2 * i2 + 1];
-                } else if (m.uv_face) {
-                    const float* uv = m.uv_face + 6 * (size_t)face;
-                    c0.u = uv[0];
-                    c0.v = uv[1];
-                    c1.u = uv[2];
-                    c1.v = uv[3];
-                    c2.u = uv[4];
-                    c2.v = uv[5];
-                }
-                if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[face] : m.mesh_tex;
-                if (p == 6) {
-                    // Every pass only rotates the triangle; after 6 passes it is
-                    // unchanged (clipping.go:199-232); no clipping: as is.
-                    mode = 1;
-                    n = 1;
-                } else {
-                    nq = clip_one_plane(p, c0, c1, c2, fr.znear, fr.zfar, q);
-                    if (nq > 0) {
-                        mode = 3;
-                        n = nq - 2;
-                    } else if (nq < 0) {
-                        nq = 0;
-                        mode = 2;  // counted below, one face at a time per wave
-                    }
-                }
-            }
-        }
-    }
-
-    // Count the fan triangles of the faces that need the full clipper.
-    CV* buf0 = wpoly[wave][0];
-    CV* buf1 = wpoly[wave][1];
-    uint64_t gen = __ballot(mode == 2);
-    for (uint64_t bits = gen; bits; bits &= bits - 1) {
-        const int L = __builtin_ctzll(bits);
-        const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
-        const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
-        if (lane == L) {
-            if (k < 0) {
-                st.overflow++;
-                mode = 0;
-                n = 0;
+                in0 = normal_intensity(o, fr, n0);
+                in1 = normal_intensity(o, fr, n1);
+                in2 = normal_intensity(o, fr, n2);
             } else {
-                n = k >= 3 ? k - 2 : 0;
+                const float* f = m.fn + 4 * (size_t)sf;
+                float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
+                in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
+            }
+        } else {
+            in0 = in1 = in2 = fr.ambient;
+        }
+        c0.i = in0;
+        c1.i = in1;
+        c2.i = in2;
+        if (m.uv_vert) {
+            c0.u = m.uv_vert[2 * i0];
+            c0.v = m.uv_vert[2 * i0 + 1];
+            c1.u = m.uv_vert[2 * i1];
+            c1.v = m.uv_vert[2 * i1 + 1];
+            c2.u = m.uv_vert[2 * i2];
+            c2.v = m.uv_vert[2 * i2 + 1];
+        } else if (m.uv_face) {
+            const float* uv = m.uv_face + 6 * (size_t)sf;
+            c0.u = uv[0];
+            c0.v = uv[1];
+            c1.u = uv[2];
+            c1.v = uv[3];
+            c2.u = uv[4];
+            c2.v = uv[5];
+        }
+        if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[sf] : m.mesh_tex;
+        if (sp == 6) {
+            // Every pass only rotates the triangle; after 6 passes it is
+            // unchanged (clipping.go:199-232); no clipping: as is.
+            mode = 1;
+            n = 1;
+        } else {
+            nq = clip_one_plane(sp, c0, c1, c2, fr.znear, fr.zfar, q);
+            if (nq > 0) {
+                mode = 3;
+                n = nq - 2;
+            } else if (nq < 0) {
+                nq = 0;
+                mode = 2;  // counted below, one face at a time per wave
             }
         }
     }
 
+    // Faces for the full clipper reserve the largest fan (kMaxFan slots): the
+    // slots their polygon leaves unused get empty records, which are never
+    // binned, so slots stay in draw order and each face is clipped once.
+    if (mode == 2) n = kMaxFan;
     int total;
     const int off = block_exclusive_scan(n, lds_scan, total);
     const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
     const uint32_t slot0 = wg_slot0 + (uint32_t)off;
-    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0));
+    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0, tile_counts, twl, thl));
     if (mode == 3) {
-        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0));
-        if (nq == 4) add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1));
+        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tile_counts, twl, thl));
+        if (nq == 4)
+            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tile_counts, twl,
+                                       thl));
     }
 
-    // Clip again (deterministic) and emit the fan with one lane per triangle.
-    gen = __ballot(mode == 2 && n > 0);
-    for (uint64_t bits = gen; bits; bits &= bits - 1) {
-        const int L = __builtin_ctzll(bits);
-        const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
-        const int nL = __shfl(n, L, 64);
-        const int texL = __shfl(tex, L, 64);
-        const uint32_t sL = (uint32_t)__shfl((int)slot0, L, 64);
-        wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
-        if (lane < nL)
-            add_emit(st, emit_triangle(fr, buf0[0], buf0[lane + 1], buf0[lane + 2], texL, sL + lane));
+    // General faces, kStage at a time per wave: the wave clips each one into
+    // its stage (clipping.go:167-236), then one lane per fan slot emits the
+    // triangle (P0, P_t+1, P_t+2) of Polygon.Triangulate (clipping.go:44-62)
+    // or an empty record.
+    CV* buf0 = wpoly[wave][0];
+    CV* buf1 = wpoly[wave][1];
+    for (uint64_t gen = __ballot(mode == 2); gen;) {
+        int nst = 0;
+        for (; gen && nst < kStage; gen &= gen - 1, nst++) {
+            const int L = __builtin_ctzll(gen);
+            const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
+            const int texL = __shfl(tex, L, 64);
+            const int sL = __shfl((int)slot0, L, 64);
+            const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
+            if (k < 0 && lane == L) st.overflow++;
+            if (lane < k) lds_stage[wave][nst][lane] = buf0[lane];
+            if (lane == 0) {
+                lds_stage_k[wave][nst] = k;
+                lds_stage_tex[wave][nst] = texL;
+                lds_stage_slot[wave][nst] = sL;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const int r = lane / kMaxFan, t = lane - r * kMaxFan;
+        if (r < nst) {
+            const int k = lds_stage_k[wave][r];
+            const uint32_t slot = (uint32_t)lds_stage_slot[wave][r] + (uint32_t)t;
+            const CV* P = lds_stage[wave][r];
+            if (t < k - 2)
+                add_emit(st, emit_triangle(fr, P[0], P[t + 1], P[t + 2], lds_stage_tex[wave][r],
+                                           slot, tile_counts, twl, thl));
+            else
+                empty_record(fr, slot);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (threadIdx.x == 0) wg_count[b] = (uint32_t)total;
 
@@ -819,48 +933,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             lds_st[wave][3] = s3;
         }
     }
-    // The barrier inside block_tile_rect also publishes this block's records
-    // to all of its threads.
-    block_tile_rect(fr.rrec, wg_slot0, (uint32_t)total, twl, thl, lrect);
+    __syncthreads();
     if (threadIdx.x < 4) {
         unsigned long long t = lds_st[0][threadIdx.x] + lds_st[1][threadIdx.x] +
                                lds_st[2][threadIdx.x] + lds_st[3][threadIdx.x];
         const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
         if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
     }
-
-    // Tile counting: LDS counters over the block's tile rectangle, one global
-    // atomic per touched tile; very large rectangles count directly.
-    const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
-    if (rx1 >= 0) {
-        const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
-        const bool in_lds = rw * rh <= kLdsTiles;
-        if (in_lds)
-            for (int i = threadIdx.x; i < rw * rh; i += 256) lcnt[i] = 0u;
-        __syncthreads();
-        for (int i = threadIdx.x; i < total; i += 256) {
-            RasterRec* rec = &fr.rrec[wg_slot0 + i];
-            set_reciprocals(rec);
-            TileRect t = rec_tiles(*rec, twl, thl);
-            for (int ty = t.ty0; ty <= t.ty1; ty++)
-                for (int tx = t.tx0; tx <= t.tx1; tx++) {
-                    if (in_lds)
-                        atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
-                    else
-                        atomicAdd(&tile_counts[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
-                }
-        }
-        __syncthreads();
-        if (in_lds)
-            for (int i = threadIdx.x; i < rw * rh; i += 256) {
-                uint32_t v = lcnt[i];
-                if (v)
-                    atomicAdd(&tile_counts[fr.tile_begin + (ry0 + i / rw) * fr.tiles_x + rx0 +
-                                           i % rw],
-                              v);
-            }
-    }
-
 }
 
 // ---------------------------------------------------------------------------
@@ -954,6 +1033,9 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
     const uint32_t cnt = wg_count[b];
     if (cnt == 0) return;
     const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256)
+        finalize_record(&fr.rrec[slot0 + i], &fr.srec[slot0 + i]);
+    // The barrier in block_tile_rect orders these writes before the reads below.
     block_tile_rect(fr.rrec, slot0, cnt, twl, thl, lrect);
     const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
     if (rx1 < 0) return;
