@@ -18,6 +18,8 @@
 // uses -ffp-contract=off and correctly rounded division/sqrt.
 #pragma clang fp contract(off)
 
+#include <cstddef>
+
 #include "gr_device.h"
 #include "gr_launch.h"
 
@@ -1181,6 +1183,13 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// int32 in the constant address space: uniform loads through it become
+// scalar loads. Only for memory no thread writes during the kernel.
+typedef const __attribute__((address_space(4))) int32_t kint;
+static_assert(offsetof(RasterRec, bx0) == 32 && offsetof(RasterRec, by0) == 36 &&
+                  offsetof(RasterRec, wide) == 40 && offsetof(RasterRec, e) == 44,
+              "scalar loads index RasterRec as int32 words");
+
 // Load of a 16-byte-aligned record through a global-address-space pointer
 // (global_load rather than flat, so LDS traffic is not waited on with it).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -1201,7 +1210,7 @@ __device__ __forceinline__ T gload(const T* p) {
 // one per lane, and the (depth, sequence) max into the tile key.
 template <int TW>
 __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t head,
-                                           uint32_t count, const RasterRec* lrec,
+                                           uint32_t count, const float4 (*ldiv)[2],
                                            const uint32_t* lslot, unsigned long long* key,
                                            int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1212,8 +1221,7 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
         const uint32_t pk = q[0][k];
         const int32_t f12 = (int32_t)q[1][k], f20 = (int32_t)q[2][k];
         const uint32_t i = pk >> 12, px = (pk >> 6) & 63u, py = pk & 63u;
-        const float4* rp = reinterpret_cast<const float4*>(&lrec[i]);
-        const Bary b = bary((float)f12, (float)f20, rp[0], rp[1]);
+        const Bary b = bary((float)f12, (float)f20, ldiv[i][0], ldiv[i][1]);
         if (b.zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(b.zrec, lslot[i] + 1u));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1224,8 +1232,7 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
 // LDS of k_raster beyond the key array: 64 staged records and slots, the
 // per-wave fragment rings and the stats reduction.
 constexpr int kRasterChunk = 64;
-constexpr int kRasterLdsFixed =
-    kRasterChunk * (int)(sizeof(RasterRec) + 4) + 4 * 3 * 128 * 4 + 4 * 2 * 8;
+constexpr int kRasterLdsFixed = kRasterChunk * (32 + 4) + 4 * 3 * 128 * 4 + 4 * 2 * 8;
 
 // Waves per SIMD the tile's LDS allows (one wave per SIMD per workgroup), at
 // most 8: the register budget follows from it (512 / waves VGPRs).
@@ -1245,9 +1252,8 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
                                                 unsigned long long* __restrict__ stats) {
     constexpr int NPX = TW * TH;
     constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
-    constexpr int PIECES = (int)(sizeof(RasterRec) / 16);
     __shared__ unsigned long long key[NPX];
-    __shared__ RasterRec lrec[CHUNK];
+    __shared__ float4 ldiv[CHUNK][2];  // divisors and reciprocals of the staged records
     __shared__ uint32_t lslot[CHUNK];
     __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
     __shared__ unsigned long long red[4][2];
@@ -1271,10 +1277,10 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
         __syncthreads();
         if (threadIdx.x < n) lslot[threadIdx.x] = refs[off + base + threadIdx.x];
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < PIECES * n; p += 256) {
-            const uint32_t ri = p / PIECES, pi = p - ri * PIECES;
+        for (uint32_t p = threadIdx.x; p < 2 * n; p += 256) {
+            const uint32_t ri = p >> 1, pi = p & 1u;
             const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[ri]]);
-            reinterpret_cast<uint4*>(&lrec[ri])[pi] = gload(src + pi);
+            reinterpret_cast<uint4*>(&ldiv[ri][0])[pi] = gload(src + pi);
         }
         __syncthreads();
         // Per wave, one triangle at a time: lanes cover an lpr x (64/lpr) pixel
@@ -1283,9 +1289,13 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
         // to an LDS ring by ballot and their exact depth runs on full waves.
         uint32_t qhead = 0, qtail = 0;  // wave-uniform ring positions
         for (uint32_t i = wave; i < n; i += 4) {
-            const RasterRec& r = lrec[i];
-            const uint32_t bxw = (uint32_t)sgpr(*reinterpret_cast<const int*>(&r.bx0));
-            const uint32_t byw = (uint32_t)sgpr(*reinterpret_cast<const int*>(&r.by0));
+            // The record's box and edge coefficients are wave-uniform: scalar
+            // loads through the constant address space (the records are
+            // read-only here).
+            const uint32_t rslot = (uint32_t)sgpr((int)lslot[i]);
+            const kint* rk = (const kint*)(const void*)&fr.rrec[rslot];
+            const uint32_t bxw = (uint32_t)rk[8];
+            const uint32_t byw = (uint32_t)rk[9];
             const int x0 = max((int)(int16_t)(bxw & 0xFFFFu), X0);
             const int x1 = min((int)(int16_t)(bxw >> 16), X0 + TW - 1);
             const int y0 = max((int)(int16_t)(byw & 0xFFFFu), Y0);
@@ -1295,11 +1305,11 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
             if (l2 > 6) l2 = 6;
             const int lpr = 1 << l2, rows = 64 >> l2;
             const int lx = lane & (lpr - 1), ly = lane >> l2;
-            if (sgpr((int)r.wide)) {
+            if (rk[10]) {
                 // Coordinates beyond the int32 range: direct int64 evaluation.
-                const uint32_t slot = lslot[i];
+                const uint32_t slot = rslot;
                 const WideRec* wr = &fr.wrec[slot];
-                const float4* rp = reinterpret_cast<const float4*>(&r);
+                const float4* rp = ldiv[i];
                 for (int yb = y0; yb <= y1; yb += rows) {
                     const int y = yb + ly;
                     for (int xb = x0; xb <= x1; xb += lpr) {
@@ -1317,9 +1327,9 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
                 }
                 continue;
             }
-            const int A0 = sgpr(r.e[0]), B0 = sgpr(r.e[1]), C0 = sgpr(r.e[2]);
-            const int A1 = sgpr(r.e[3]), B1 = sgpr(r.e[4]), C1 = sgpr(r.e[5]);
-            const int A2 = sgpr(r.e[6]), B2 = sgpr(r.e[7]), C2 = sgpr(r.e[8]);
+            const int A0 = rk[11], B0 = rk[12], C0 = rk[13];
+            const int A1 = rk[14], B1 = rk[15], C1 = rk[16];
+            const int A2 = rk[17], B2 = rk[18], C2 = rk[19];
             // |A|, |B| <= 2^15 and |x|, |y| < 2^14 on this path: 24-bit products
             // are exact and every edge value fits int32.
             const int px = x0 + lx, py = y0 + ly;
@@ -1355,7 +1365,7 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
                         qtail += m;
                         covered_wave += m;
                         if (qtail - qhead >= 64u) {
-                            frag_depth<TW>(q, qhead, 64u, lrec, lslot, key, lane);
+                            frag_depth<TW>(q, qhead, 64u, ldiv, lslot, key, lane);
                             qhead += 64u;
                         }
                     }
@@ -1372,7 +1382,7 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
         }
         while (qtail != qhead) {
             const uint32_t cnt_q = min(qtail - qhead, 64u);
-            frag_depth<TW>(q, qhead, cnt_q, lrec, lslot, key, lane);
+            frag_depth<TW>(q, qhead, cnt_q, ldiv, lslot, key, lane);
             qhead += cnt_q;
         }
     }
@@ -1420,7 +1430,25 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
                 f12f = (float)f[1];
                 f20f = (float)f[2];
             }
-            const Bary b = bary(f12f, f20f, rp[0], rp[1]);
+            // The key holds zRec's bits (with -0 folded into +0): only a zero
+            // zRec is recomputed; alpha and beta are as in bary().
+            const uint32_t ord = (uint32_t)(k >> 32);
+            const float zkey = __uint_as_float((ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord);
+            Bary b;
+            if (zkey != 0.0f) {
+                const float4 d = rp[0], rc = rp[1];
+                if (rc.x != 0.f) {
+                    b.alpha = div_rcp(f12f, d.x, rc.x);
+                    b.beta = div_rcp(f20f, d.x, rc.x);
+                } else {
+                    b.alpha = f12f / d.x;
+                    b.beta = f20f / d.x;
+                }
+                b.gamma = 1.0f - b.alpha - b.beta;
+                b.zrec = zkey;
+            } else {
+                b = bary(f12f, f20f, rp[0], rp[1]);
+            }
             // rasterizer.go:158-170
             float u = (b.alpha * s.uz[0] + b.beta * s.uz[1] + b.gamma * s.uz[2]) / b.zrec;
             float v = (b.alpha * s.vz[0] + b.beta * s.vz[1] + b.gamma * s.vz[2]) / b.zrec;
