@@ -223,7 +223,7 @@ def main():
     avg_launch_s = dom_ms / dom_launches / 1e3
     alg_bytes = {
         "k_raster": W * H * 8 + 16 * 16 * 4,
-        "k_face": faces * 12 + scene.meshes[0].num_vertices * 8,
+        "k_face": faces * 12 + scene.meshes[0].num_vertices * (12 + 12 + 8),
         "k_vertex": scene.meshes[0].num_vertices * 24,
         "k_bin_fill": 0,
         "k_bin_scan": 0,

@@ -713,7 +713,7 @@ __device__ __forceinline__ float normal_intensity(const DevObject& o, const DevF
     return lambert(xform(o.world, q[0], q[1], q[2], w), fr.light, fr.ambient, fr.diffuse);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_face(const DevObject* __restrict__ objs, int nobj,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_face(const DevObject* __restrict__ objs, int nobj,
                                               const DevFrame* __restrict__ frames,
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
