@@ -85,6 +85,8 @@ struct gr_context {
     // batch: frames | objects | stats | counters | frame_overflow,
     // uploaded with one copy per draw (the counters as zeros).
     DevBuf batch, rrec, srec, wrec, tile_counts, tiles, wgcount, refs, out_rgba, out_depth, ctrl;
+    DevBuf okey;  // overlay keys of the frames that draw overlays; zero between draws
+    DevBuf orec;  // overlay records per slot (batches with overlay frames)
     Staging ring[4];
     int ring_i = 0;
     void* h_ctrl = nullptr;   // pinned: sticky control words
@@ -236,8 +238,7 @@ bool valid_tile(int v) { return v == 8 || v == 16 || v == 32 || v == 64; }
 gr_status validate_frame(const gr_frame& f) {
     if (f.width < 1 || f.height < 1 || f.width > gr::kMaxDim || f.height > gr::kMaxDim)
         return fail(GR_ERR_INVALID_ARGUMENT, "frame size must be within 1..16384");
-    if (f.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES))
-        return fail(GR_ERR_UNSUPPORTED, "edge/vertex overlays are not implemented on the GPU path");
+
     if (f.ref_tiles != 1 && f.ref_tiles != 16)
         return fail(GR_ERR_INVALID_ARGUMENT, "ref_tiles must be 1 or 16");
     return GR_OK;
@@ -273,7 +274,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     std::vector<gr::DevObject> dobj;
     dobj.reserve(no);
     int fblocks = 0, tiles = 0;
-    size_t slots = 0;
+    size_t slots = 0, okey_words = 0;
     for (int f = 0; f < nf; f++) {
         const gr_frame& fr = bf[f].frame;
         gr_status st = validate_frame(fr);
@@ -308,6 +309,12 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         d.face_block_begin = fblocks;
         d.rgba = (uint8_t*)bf[f].d_rgba;
         d.depth = (float*)bf[f].d_depth;
+        // Overlay key buffer offset (bytes + 1; 0 = no overlays).
+        d.okey = nullptr;
+        if (fr.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES)) {
+            d.okey = (uint32_t*)(uintptr_t)(okey_words * 4 + 1);
+            okey_words += (size_t)fr.width * fr.height;
+        }
         int seq = 0;
         for (int k = 0; k < bf[f].num_objects; k++) {
             const gr_object& ob = objs[bf[f].first_object + k];
@@ -337,6 +344,9 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         d.face_blocks = fblocks - d.face_block_begin;
         if ((size_t)d.face_blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
             return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
+        if (d.okey && (size_t)d.face_blocks * gr::kSlotsPerWg >= 0x7FFFFFFFull)
+            return fail(GR_ERR_UNSUPPORTED, "overlays on a frame with more than 2^31 slots");
+        d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)(slots * sizeof(gr::OverlayRec) + 1) : nullptr;
         d.rrec = (gr::RasterRec*)(uintptr_t)(slots * sizeof(gr::RasterRec));
         d.srec = (gr::ShadeRec*)(uintptr_t)(slots * sizeof(gr::ShadeRec));
         d.wrec = (gr::WideRec*)(uintptr_t)(slots * sizeof(gr::WideRec));
@@ -389,9 +399,20 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         if ((st = ensure(c, c->ctrl, sizeof(gr::DevControl)))) return st;
         GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
     }
+    // Overlay keys are zero between draws (k_raster re-zeroes what it reads);
+    // a new allocation starts zeroed.
+    if (okey_words && (st = ensure(c, c->orec, slots * sizeof(gr::OverlayRec)))) return st;
+    if (okey_words && c->okey.cap < okey_words * 4) {
+        void* old = c->okey.p;
+        if ((st = ensure(c, c->okey, okey_words * 4))) return st;
+        if (c->okey.p != old) GR_HIP(hipMemsetAsync(c->okey.p, 0, c->okey.cap, c->stream));
+    }
 
     // Offsets -> device pointers, then one upload of the whole batch header.
     for (auto& d : frames) {
+        if (d.okey) d.okey = (uint32_t*)((char*)c->okey.p + ((uintptr_t)d.okey - 1));
+        if (d.orec) d.orec = (gr::OverlayRec*)((char*)c->orec.p + ((uintptr_t)d.orec - 1));
+        d.ctrl = (gr::DevControl*)c->ctrl.p;
         d.rrec = (gr::RasterRec*)((char*)c->rrec.p + (uintptr_t)d.rrec);
         d.srec = (gr::ShadeRec*)((char*)c->srec.p + (uintptr_t)d.srec);
         d.wrec = (gr::WideRec*)((char*)c->wrec.p + (uintptr_t)d.wrec);
@@ -435,6 +456,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     timed_launch(c, "k_face", [&] { gr::launch_face(p, c->stream); });
     timed_launch(c, "k_bin_scan", [&] { gr::launch_bin_scan(p, c->stream); });
     timed_launch(c, "k_bin_fill", [&] { gr::launch_bin_fill(p, c->stream); });
+    if (okey_words) timed_launch(c, "k_overlay", [&] { gr::launch_overlay(p, c->stream); });
     bool ok = true;
     timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
     hipError_t le = hipGetLastError();
@@ -478,6 +500,11 @@ gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
     gr::DevControl ctl;
     std::memcpy(&ctl, c->h_ctrl, sizeof ctl);
     *needed = 0;
+    if (ctl.unsupported) {
+        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
+        GR_HIP(hipStreamSynchronize(c->stream));
+        return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+    }
     if (ctl.overflow) {
         *needed = ctl.refs_needed;
         GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
@@ -558,7 +585,7 @@ void gr_destroy(gr_context* c) {
     for (auto& t : c->textures)
         if (t.texels) (void)hipFree(t.texels);
     DevBuf* bufs[] = {&c->d_textures, &c->batch, &c->rrec, &c->srec, &c->wrec,
-                      &c->tile_counts, &c->tiles, &c->wgcount, &c->refs, &c->out_rgba,
+                      &c->tile_counts, &c->tiles, &c->wgcount, &c->refs, &c->out_rgba, &c->okey, &c->orec,
                       &c->out_depth, &c->ctrl};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);

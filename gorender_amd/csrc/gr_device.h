@@ -33,6 +33,8 @@ enum : uint32_t {
     kFlatShading = 1u << 3,
     kFrustumClipping = 1u << 4,
     kShowTextures = 1u << 5,
+    kShowEdges = 1u << 6,
+    kShowVertices = 1u << 7,
 };
 
 struct DevMesh {
@@ -70,6 +72,8 @@ struct DevTexture {
     const uint32_t* texels;
 };
 
+struct DevControl;
+
 struct DevFrame {
     int32_t W, H;
     uint32_t flags;
@@ -87,6 +91,9 @@ struct DevFrame {
     int32_t pad0;
     uint8_t* rgba;               // device outputs
     float* depth;
+    uint32_t* okey;              // overlay keys [H*W] or null: (slot+1) << 1 | vertex
+    struct OverlayRec* orec;     // per slot when okey is set
+    DevControl* ctrl;
     struct RasterRec* rrec;      // frame-local slot arrays
     struct ShadeRec* srec;
     struct WideRec* wrec;
@@ -112,6 +119,13 @@ struct alignas(16) ShadeRec {
     float in[3];  // intensities
     int32_t tex;  // texture slot or -1
     int32_t pad[2];
+};
+
+// Screen positions of a post-clip triangle for the debug overlays, 32 bytes.
+struct alignas(16) OverlayRec {
+    float sx[3], sy[3];
+    uint32_t binned;  // drawn by drawProjection (renderer.go:166-217)
+    uint32_t pad;
 };
 
 // Exact int64 coordinates for triangles outside the int32 fast range.
@@ -145,7 +159,10 @@ struct DevBatchCounters {
 struct DevControl {
     unsigned long long refs_needed;
     uint32_t overflow;
-    uint32_t pad;
+    uint32_t unsupported;  // an overlay line longer than kMaxOverlayLine
 };
+
+// Longest overlay line drawn (pixels); longer ones make the draw unsupported.
+constexpr int64_t kMaxOverlayLine = 1 << 16;
 
 }  // namespace gr

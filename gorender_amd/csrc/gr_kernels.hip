@@ -390,6 +390,80 @@ struct FaceStats {
     unsigned long long tpf, tris, front, overflow;
 };
 
+// ---------------------------------------------------------------------------
+// Debug overlays (renderer.go:191-216): lines and 3x3 rects write colour only,
+// after their triangle's face, in draw order. Each overlay pixel keeps the max
+// of (slot + 1) << 1 | vertex; the raster's shading pass shows the overlay
+// where it comes at or after the depth winner (within a triangle the vertex
+// rects are drawn last).
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kEdgeColor = 0xFF000000u;    // {0,0,0,255} (renderer.go:19)
+constexpr uint32_t kVertexColor = 0xFF00A1FFu;  // {255,161,0,255} (renderer.go:18)
+constexpr uint32_t kWhite = 0xFFFFFFFFu;        // edges without faces (renderer.go:194)
+
+// utils.go:11-21: minInt maps to maxInt.
+__device__ __forceinline__ int64_t go_abs(int64_t v) {
+    if (v < 0) v = wsub(0, v);
+    if (v < 0) v = wsub(v, 1);
+    return v;
+}
+
+// FrameBuffer.Pixel (rasterizer.go:25-30): only the flat index is checked.
+__device__ __forceinline__ void ov_pixel(const DevFrame& fr, int64_t x, int64_t y, uint32_t code) {
+    const int64_t idx = wadd(wmul(y, (int64_t)fr.W), x);
+    if (idx > 0 && idx < (int64_t)fr.W * fr.H) atomicMax(&fr.okey[idx], code);
+}
+
+// FrameBuffer.Rect (rasterizer.go:54-63) with width = height = 3.
+__device__ void ov_rect(const DevFrame& fr, int64_t x, int64_t y, uint32_t code) {
+    if (x >= fr.W || y >= fr.H) return;
+    for (int64_t py = y; py < wadd(y, 3); py = wadd(py, 1))
+        for (int64_t px = x; px < wadd(x, 3); px = wadd(px, 1)) ov_pixel(fr, px, py, code);
+}
+
+// FrameBuffer.Line (rasterizer.go:65-79): a float32 DDA accumulated step by step.
+__device__ void ov_line(const DevFrame& fr, int64_t x0, int64_t y0, int64_t x1, int64_t y1,
+                        uint32_t code) {
+    const int64_t dx = wsub(x1, x0), dy = wsub(y1, y0);
+    const int64_t ax = go_abs(dx), ay = go_abs(dy);
+    const int64_t side = ax > ay ? ax : ay;
+    if (side > kMaxOverlayLine) {
+        atomicOr(&fr.ctrl->unsupported, 1u);
+        return;
+    }
+    const float xs = (float)dx / (float)side, ys = (float)dy / (float)side;
+    float cx = (float)x0, cy = (float)y0;
+    for (int64_t i = 0; i <= side; i++) {
+        ov_pixel(fr, go_int(cx), go_int(cy), code);
+        cx += xs;
+        cy += ys;
+    }
+}
+
+// drawProjection's overlays for one binned triangle (renderer.go:191-216).
+__device__ void draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t slot) {
+    const float* sx = o.sx;
+    const float* sy = o.sy;
+    int64_t ix[3], iy[3];
+    for (int j = 0; j < 3; j++) {
+        ix[j] = go_int(sx[j]);
+        iy[j] = go_int(sy[j]);
+    }
+    const uint32_t code = (slot + 1u) << 1;
+    if (fr.flags & kShowEdges) {
+        ov_line(fr, ix[0], iy[0], ix[1], iy[1], code);
+        ov_line(fr, ix[1], iy[1], ix[2], iy[2], code);
+        ov_line(fr, ix[2], iy[2], ix[0], iy[0], code);
+        if (fr.flags & kShowFaces) {
+            const float mx = (sx[0] + sx[1] + sx[2]) / 3.0f, my = (sy[0] + sy[1] + sy[2]) / 3.0f;
+            ov_rect(fr, wsub(go_int(mx), 1), wsub(go_int(my), 1), code);
+        }
+    }
+    if (fr.flags & kShowVertices)
+        for (int j = 0; j < 3; j++) ov_rect(fr, wsub(ix[j], 1), wsub(iy[j], 1), code | 1u);
+}
+
 // Writes the set-up of one post-clip triangle into its slot (renderer.go:361-394
 // then the per-triangle part of rasterizer.go:98-137). Returns the TPF
 // references in the low 16 bits and 1 << 16 when the triangle was binned.
@@ -449,6 +523,18 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
     for (int e = 0; e < 9; e++) rr.e[e] = 0;
     float W = (float)fr.W, H = (float)fr.H;
     bool binned = fmaxx >= 0.f && fminx <= W && fmaxy >= 0.f && fminy <= H;
+    if (fr.orec) {
+        // Screen positions for the overlay pass (binned triangles only draw them).
+        OverlayRec o;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            o.sx[j] = sx[j];
+            o.sy[j] = sy[j];
+        }
+        o.binned = binned ? 1u : 0u;
+        o.pad = 0;
+        fr.orec[slot] = o;
+    }
     if (binned) {
         tris = 1;
         int64_t ix[3], iy[3];
@@ -570,6 +656,10 @@ __device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t slot) 
 #pragma unroll
     for (int e = 0; e < 9; e++) rr.e[e] = 0;
     fr.rrec[slot] = rr;
+    if (fr.orec) {
+        OverlayRec o = {};
+        fr.orec[slot] = o;
+    }
 }
 
 // General faces clipped per round by one wave; each fills kMaxFan lanes.
@@ -941,6 +1031,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                lds_st[2][threadIdx.x] + lds_st[3][threadIdx.x];
         const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
         if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_overlay: debug overlays of every binned triangle (frames that ask for them)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ objs, int nobj,
+                                                 const DevFrame* __restrict__ frames,
+                                                 const uint32_t* __restrict__ wg_count) {
+    const int b = blockIdx.x;
+    const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
+    const DevObject& o = objs[oi];
+    const DevFrame& fr = frames[o.frame];
+    if (!fr.orec) return;
+    const uint32_t cnt = wg_count[b];
+    const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+        const OverlayRec rec = fr.orec[slot0 + i];
+        if (rec.binned) draw_overlays(fr, rec, slot0 + i);
     }
 }
 
@@ -1458,6 +1568,15 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
             depth = b.zrec;
         }
         const size_t idx = (size_t)y * fr.W + x;
+        if (fr.okey) {
+            // Overlay keys are zero between draws: each pixel's is read once.
+            const uint32_t ok = fr.okey[idx];
+            if (ok) {
+                fr.okey[idx] = 0u;
+                if ((ok >> 1) >= seq)
+                    color = (ok & 1u) ? kVertexColor : ((fr.flags & kShowFaces) ? kEdgeColor : kWhite);
+            }
+        }
         ((uint32_t*)fr.rgba)[idx] = color;
         fr.depth[idx] = depth;
     }
@@ -1526,6 +1645,12 @@ void xcd_tables(const int* n, int nframes, int* table, int* grid) {
         longest = std::max(longest, P[nframes]);
     }
     *grid = 8 * longest;
+}
+
+void launch_overlay(const LaunchParams& p, hipStream_t s) {
+    if (p.face_blocks == 0) return;
+    hipLaunchKernelGGL(k_overlay, dim3(p.face_blocks), dim3(256), 0, s, p.objs, p.nobj, p.frames,
+                       p.wg_count);
 }
 
 void launch_bin_scan(const LaunchParams& p, hipStream_t s) {

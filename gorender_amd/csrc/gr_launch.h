@@ -45,6 +45,7 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 void launch_check_div(uint64_t n, uint64_t seed, unsigned long long* mismatches, float* first_bad,
                       hipStream_t s);
 void launch_bin_scan(const LaunchParams& p, hipStream_t s);
+void launch_overlay(const LaunchParams& p, hipStream_t s);
 void launch_bin_fill(const LaunchParams& p, hipStream_t s);
 bool launch_raster(const LaunchParams& p, hipStream_t s);
 

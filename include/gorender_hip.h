@@ -61,8 +61,10 @@ enum {
     GR_FLAT_SHADING = 1u << 3,
     GR_FRUSTUM_CLIPPING = 1u << 4,
     GR_SHOW_TEXTURES = 1u << 5,
-    GR_SHOW_EDGES = 1u << 6,    /* debug overlay (renderer.go:191-210): GR_ERR_UNSUPPORTED */
-    GR_SHOW_VERTICES = 1u << 7, /* debug overlay (renderer.go:212-216): GR_ERR_UNSUPPORTED */
+    GR_SHOW_EDGES = 1u << 6,    /* debug overlay: triangle edges (renderer.go:191-210) */
+    GR_SHOW_VERTICES = 1u << 7, /* debug overlay: vertex dots (renderer.go:212-216); an
+                                   overlay line longer than 65536 pixels (unclipped
+                                   geometry far off screen) -> GR_ERR_UNSUPPORTED */
 };
 #define GR_DEFAULT_FLAGS \
     (GR_SHOW_FACES | GR_BACKFACE_CULLING | GR_LIGHTING | GR_FRUSTUM_CLIPPING | GR_SHOW_TEXTURES)

@@ -671,6 +671,44 @@ void rc_dot_grid(int32_t width, int32_t height, uint8_t *rgba, const uint8_t c[4
         }
 }
 
+/* rasterizer.go:25-30: only the flat index is checked, so x outside the row
+ * wraps into a neighbouring row, and index 0 is never written. */
+void rc_pixel(int32_t width, int32_t height, uint8_t *rgba, int64_t x, int64_t y,
+              const uint8_t c[4]) {
+    int64_t idx = wadd(wmul(y, width), x);
+    if (idx > 0 && idx < (int64_t)width * height) memcpy(rgba + 4 * idx, c, 4);
+}
+
+/* utils.go:11-21: minInt maps to maxInt. */
+int64_t rc_go_abs(int64_t v) {
+    if (v < 0) v = wsub(0, v);
+    if (v < 0) v = wsub(v, 1);
+    return v;
+}
+
+/* rasterizer.go:54-63 */
+void rc_rect(int32_t width, int32_t height, uint8_t *rgba, int64_t x, int64_t y, int64_t w,
+             int64_t h, const uint8_t c[4]) {
+    if (x >= width || y >= height) return;
+    for (int64_t py = y; py < wadd(y, h); py = wadd(py, 1))
+        for (int64_t px = x; px < wadd(x, w); px = wadd(px, 1)) rc_pixel(width, height, rgba, px, py, c);
+}
+
+/* rasterizer.go:65-79: a float32 DDA accumulated step by step. */
+void rc_line(int32_t width, int32_t height, uint8_t *rgba, int64_t x0, int64_t y0, int64_t x1,
+             int64_t y1, const uint8_t c[4]) {
+    int64_t dx = wsub(x1, x0), dy = wsub(y1, y0);
+    int64_t ax = rc_go_abs(dx), ay = rc_go_abs(dy);
+    int64_t side = ax > ay ? ax : ay;
+    float xs = (float)dx / (float)side, ys = (float)dy / (float)side;
+    float cx = (float)x0, cy = (float)y0;
+    for (int64_t i = 0; i <= side; i++) {
+        rc_pixel(width, height, rgba, rc_go_int(cx), rc_go_int(cy), c);
+        cx += xs;
+        cy += ys;
+    }
+}
+
 /* renderer.go:50-76 */
 void rc_tile_bounds(int32_t tile, int32_t num_tiles, int32_t width, int32_t height, rc_vec2 *start,
                     rc_vec2 *end) {
@@ -888,23 +926,48 @@ done:
     free(wvn);
 }
 
+/* renderer.go:18-19 */
+static const uint8_t kEdgeColor[4] = {0, 0, 0, 255};
+static const uint8_t kVertexColor[4] = {255, 161, 0, 255};
+static const uint8_t kWhite[4] = {255, 255, 255, 255};
+
+/* drawProjection (renderer.go:166-217) over a tile list in order: the face,
+ * then the edge and vertex overlays. */
 static void raster_list(const rc_frame *fr, const rc_texture *textures, const rc_trilist *l,
                         int64_t tsx, int64_t tsy, int64_t tex, int64_t tey, uint8_t *rgba,
                         float *depth, int64_t *covered, int64_t *passed, uint8_t *written) {
     int show_tex = (fr->flags & RC_SHOW_TEXTURES) != 0;
+    int W = fr->width, H = fr->height;
     for (int64_t i = 0; i < l->n; i++) {
         const rc_tri *t = &l->v[i];
         const rc_vec4 *a = &t->points[0], *b = &t->points[1], *c = &t->points[2];
         const rc_texture *tx = (show_tex && t->texture >= 0) ? &textures[t->texture] : NULL;
-        int64_t p = 0;
-        *covered += rc_triangle(fr->width, fr->height, rgba, depth, rc_go_int(a->x),
-                                rc_go_int(a->y), a->w, t->uvs[0].u, t->uvs[0].v,
-                                rc_go_int(b->x), rc_go_int(b->y), b->w, t->uvs[1].u,
-                                t->uvs[1].v, rc_go_int(c->x), rc_go_int(c->y), c->w,
-                                t->uvs[2].u, t->uvs[2].v, tsx, tsy, tex, tey, t->intensity[0],
-                                t->intensity[1], t->intensity[2], tx, fr->face_color, &p,
-                                written);
-        *passed += p;
+        int64_t ax = rc_go_int(a->x), ay = rc_go_int(a->y), bx = rc_go_int(b->x),
+                by = rc_go_int(b->y), cx = rc_go_int(c->x), cy = rc_go_int(c->y);
+        if (fr->flags & RC_SHOW_FACES) {
+            int64_t p = 0;
+            *covered += rc_triangle(W, H, rgba, depth, ax, ay, a->w, t->uvs[0].u, t->uvs[0].v,
+                                    bx, by, b->w, t->uvs[1].u, t->uvs[1].v, cx, cy, c->w,
+                                    t->uvs[2].u, t->uvs[2].v, tsx, tsy, tex, tey,
+                                    t->intensity[0], t->intensity[1], t->intensity[2], tx,
+                                    fr->face_color, &p, written);
+            *passed += p;
+        }
+        if (fr->flags & RC_SHOW_EDGES) {
+            const uint8_t *col = (fr->flags & RC_SHOW_FACES) ? kEdgeColor : kWhite;
+            rc_line(W, H, rgba, ax, ay, bx, by, col);
+            rc_line(W, H, rgba, bx, by, cx, cy, col);
+            rc_line(W, H, rgba, cx, cy, ax, ay, col);
+            if (fr->flags & RC_SHOW_FACES) {
+                float mx = (a->x + b->x + c->x) / 3.0f, my = (a->y + b->y + c->y) / 3.0f;
+                rc_rect(W, H, rgba, wsub(rc_go_int(mx), 1), wsub(rc_go_int(my), 1), 3, 3, col);
+            }
+        }
+        if (fr->flags & RC_SHOW_VERTICES) {
+            rc_rect(W, H, rgba, wsub(ax, 1), wsub(ay, 1), 3, 3, kVertexColor);
+            rc_rect(W, H, rgba, wsub(bx, 1), wsub(by, 1), 3, 3, kVertexColor);
+            rc_rect(W, H, rgba, wsub(cx, 1), wsub(cy, 1), 3, 3, kVertexColor);
+        }
     }
 }
 
@@ -954,7 +1017,7 @@ int rc_draw(const rc_frame *fr, const rc_mesh *meshes, int32_t nm, const rc_text
         }
     }
     s.triangles = list.n;
-    if (fr->flags & RC_SHOW_FACES) {
+    if (fr->flags & (RC_SHOW_FACES | RC_SHOW_EDGES | RC_SHOW_VERTICES)) {
         uint8_t *written = (uint8_t *)calloc((size_t)fr->width * fr->height, 1);
         if (!written) {
             free(list.v);
@@ -1023,6 +1086,8 @@ int rc_draw_parallel(const rc_frame *fr, const rc_mesh *meshes, int32_t nm,
                      int32_t tiles_x, int32_t tiles_y) {
     int err = validate(fr, meshes, nm, ntex, objs, no);
     if (err) return err;
+    /* Overlays write outside their tile (Pixel wraps rows): sequential form only. */
+    if (fr->flags & (RC_SHOW_EDGES | RC_SHOW_VERTICES)) return 3;
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
     if (tiles_x < 1) tiles_x = 1;

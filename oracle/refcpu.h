@@ -70,6 +70,8 @@ enum {
     RC_FLAT_SHADING = 1u << 3,
     RC_FRUSTUM_CLIPPING = 1u << 4,
     RC_SHOW_TEXTURES = 1u << 5,
+    RC_SHOW_EDGES = 1u << 6,
+    RC_SHOW_VERTICES = 1u << 7,
 };
 
 typedef struct {
@@ -151,6 +153,14 @@ int64_t rc_triangle(int32_t width, int32_t height, uint8_t *rgba, float *depth, 
                     int64_t *passed, uint8_t *written);
 void rc_clear(int32_t width, int32_t height, uint8_t *rgba, float *depth, const uint8_t c[4]);
 void rc_dot_grid(int32_t width, int32_t height, uint8_t *rgba, const uint8_t c[4], int32_t step);
+/* FrameBuffer.Pixel / Rect / Line (rasterizer.go:25-30,54-79), abs (utils.go:11-21). */
+void rc_pixel(int32_t width, int32_t height, uint8_t *rgba, int64_t x, int64_t y,
+              const uint8_t c[4]);
+int64_t rc_go_abs(int64_t v);
+void rc_rect(int32_t width, int32_t height, uint8_t *rgba, int64_t x, int64_t y, int64_t w,
+             int64_t h, const uint8_t c[4]);
+void rc_line(int32_t width, int32_t height, uint8_t *rgba, int64_t x0, int64_t y0, int64_t x1,
+             int64_t y1, const uint8_t c[4]);
 void rc_tile_bounds(int32_t tile, int32_t num_tiles, int32_t width, int32_t height, rc_vec2 *start,
                     rc_vec2 *end);
 

@@ -117,6 +117,10 @@ def load() -> ctypes.CDLL:
                                   POINTER(c_int64)]),
         "rc_clear": (None, [c_int32, c_int32, P, P, P]),
         "rc_dot_grid": (None, [c_int32, c_int32, P, P, c_int32]),
+        "rc_pixel": (None, [c_int32, c_int32, P, c_int64, c_int64, P]),
+        "rc_go_abs": (c_int64, [c_int64]),
+        "rc_rect": (None, [c_int32, c_int32, P, c_int64, c_int64, c_int64, c_int64, P]),
+        "rc_line": (None, [c_int32, c_int32, P, c_int64, c_int64, c_int64, c_int64, P]),
         "rc_tile_bounds": (None, [c_int32, c_int32, c_int32, c_int32, POINTER(rc_vec2),
                                   POINTER(rc_vec2)]),
         "rc_draw": (c_int32, [POINTER(rc_frame), P, c_int32, P, c_int32, P, c_int32, P, P,

@@ -61,6 +61,16 @@ def overlapping_objects(w=128, h=96, seed=2):
     return S.SceneData(fr, [m0, m1, m2], [t0, t1, t2], objs, name="overlap")
 
 
+def no(bit):
+    """The default toggles without `bit`."""
+    return ALL & ~bit
+
+
+def with_flags(scene, flags):
+    scene.frame.flags = flags
+    return scene
+
+
 def inside_sphere(flags=ALL, znear=0.0, w=128, h=96, seed=3):
     """Camera inside a sphere (faces wound inward) looking along -z: near-plane
     and side clipping of triangles that pass behind the camera."""
@@ -151,7 +161,6 @@ def degenerate(w=64, h=48):
 
 def all_cases():
     """(name, builder) pairs of the small parity suite."""
-    no = lambda bit: ALL & ~bit  # noqa: E731
     return [
         ("sphere-default", lambda: sphere()),
         ("sphere-near", lambda: sphere(eye=(0, 0, 1.15), nlat=24, nlon=40)),
@@ -182,6 +191,15 @@ def all_cases():
         ("soup-wide", lambda: clip_space_soup(flags=no(_lib.GR_FRUSTUM_CLIPPING), seed=7,
                                               w_range=(-1e-4, 1e-4))),
         ("degenerate", lambda: degenerate()),
+        ("overlay-edges", lambda: sphere(flags=ALL | _lib.GR_SHOW_EDGES)),
+        ("overlay-vertices", lambda: sphere(flags=ALL | _lib.GR_SHOW_VERTICES, seed=3)),
+        ("overlay-both-nofaces", lambda: sphere(
+            flags=no(_lib.GR_SHOW_FACES) | _lib.GR_SHOW_EDGES | _lib.GR_SHOW_VERTICES, seed=4)),
+        ("overlay-soup", lambda: clip_space_soup(
+            flags=ALL | _lib.GR_SHOW_EDGES | _lib.GR_SHOW_VERTICES, seed=8)),
+        ("overlay-ties", lambda: with_flags(overlapping_objects(), ALL | _lib.GR_SHOW_EDGES)),
+        ("overlay-inside", lambda: inside_sphere(flags=ALL | _lib.GR_SHOW_EDGES
+                                                 | _lib.GR_SHOW_VERTICES)),
     ]
 
 

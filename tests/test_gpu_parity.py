@@ -130,11 +130,16 @@ def test_errors(ctx):
     # ... but flat shading never reads the normal indices, as in the reference
     scene.frame.flags |= _lib.GR_FLAT_SHADING
     ctx.draw(scene, [h])
-    scene.frame.flags |= _lib.GR_SHOW_EDGES
+    # Overlay lines of unclipped geometry far off screen would walk billions of
+    # pixels in the reference; the GPU path reports them as unsupported and the
+    # context stays usable.
+    import scene_cases as C
+    wide = C.clip_space_soup(flags=C.no(_lib.GR_FRUSTUM_CLIPPING) | _lib.GR_SHOW_EDGES, seed=7,
+                             w_range=(-1e-4, 1e-4))
     with pytest.raises(_lib.GorenderError) as e:
-        ctx.draw(scene, [h])
+        ctx.draw(wide)
     assert e.value.status == _lib.GR_ERR_UNSUPPORTED
-    scene.frame.flags &= ~_lib.GR_SHOW_EDGES
+    ctx.draw(scene, [h])
     with pytest.raises(_lib.GorenderError) as e:
         ctx.draw(scene, [h + 1000])
     assert e.value.status == _lib.GR_ERR_INVALID_HANDLE

@@ -585,3 +585,70 @@ def test_tile_bounds(lib):
     assert (s.x, s.y, e.x, e.y) == (201, 151, 402, 302)
     lib.rc_tile_bounds(15, 16, 801, 601, ctypes.byref(s), ctypes.byref(e))
     assert (s.x, s.y, e.x, e.y) == (603, 453, 801, 601)
+
+
+# ---- overlays: FrameBuffer.Pixel / Rect / Line (rasterizer.go:25-30,54-79), abs (utils.go) ----
+
+def _overlay_buf(w, h):
+    return np.zeros((h, w, 4), np.uint8)
+
+
+def _written(buf):
+    return {(int(x), int(y)) for y, x in zip(*np.nonzero(buf[..., 3]))}
+
+
+def test_go_abs_and_pixel_wrap():
+    lib = refcpu.load()
+    assert lib.rc_go_abs(-5) == 5 and lib.rc_go_abs(7) == 7
+    assert lib.rc_go_abs(-(1 << 63)) == (1 << 63) - 1  # minInt -> maxInt
+    w, h = 5, 4
+    c = (ctypes.c_uint8 * 4)(1, 2, 3, 255)
+    buf = _overlay_buf(w, h)
+    p = buf.ctypes.data_as(ctypes.c_void_p)
+    lib.rc_pixel(w, h, p, 0, 0, c)      # index 0 is never written
+    lib.rc_pixel(w, h, p, -1, 1, c)     # x = -1 wraps to the end of row 0
+    lib.rc_pixel(w, h, p, w, 1, c)      # x = W wraps to the start of row 2
+    lib.rc_pixel(w, h, p, 0, h, c)      # past the end: nothing
+    lib.rc_pixel(w, h, p, -2, 0, c)     # negative index: nothing
+    assert _written(buf) == {(w - 1, 0), (0, 2)}
+
+
+def test_rect_closed_forms():
+    lib = refcpu.load()
+    w, h = 8, 6
+    c = (ctypes.c_uint8 * 4)(9, 9, 9, 255)
+    buf = _overlay_buf(w, h)
+    p = buf.ctypes.data_as(ctypes.c_void_p)
+    lib.rc_rect(w, h, p, 2, 1, 3, 3, c)
+    assert _written(buf) == {(x, y) for x in (2, 3, 4) for y in (1, 2, 3)}
+    buf[:] = 0
+    lib.rc_rect(w, h, p, w, 0, 3, 3, c)   # x >= W: returns before drawing
+    lib.rc_rect(w, h, p, 0, h, 3, 3, c)   # y >= H
+    assert _written(buf) == set()
+    lib.rc_rect(w, h, p, -1, 2, 3, 3, c)  # x = -1 wraps into the previous row
+    # columns -1, 0, 1 of rows 2..4: column -1 is the last pixel of the row above
+    assert _written(buf) == {(w - 1, 1), (0, 2), (1, 2), (w - 1, 2), (0, 3), (1, 3),
+                             (w - 1, 3), (0, 4), (1, 4)}
+
+
+def test_line_closed_forms():
+    lib = refcpu.load()
+    w, h = 16, 12
+    c = (ctypes.c_uint8 * 4)(5, 5, 5, 255)
+    buf = _overlay_buf(w, h)
+    p = buf.ctypes.data_as(ctypes.c_void_p)
+    lib.rc_line(w, h, p, 1, 2, 6, 2, c)   # horizontal, both ends included
+    assert _written(buf) == {(x, 2) for x in range(1, 7)}
+    buf[:] = 0
+    lib.rc_line(w, h, p, 3, 9, 3, 4, c)   # vertical, drawn from the first end
+    assert _written(buf) == {(3, y) for y in range(4, 10)}
+    buf[:] = 0
+    lib.rc_line(w, h, p, 0, 0, 8, 2, c)   # step 2/8 = 0.25 exactly: y = floor(i / 4)
+    # (0, 0) is flat index 0, which Pixel never writes
+    assert _written(buf) == {(i, i // 4) for i in range(1, 9)}
+    buf[:] = 0
+    lib.rc_line(w, h, p, 4, 4, 4, 4, c)   # a point: side 0, one pixel (then NaN steps)
+    assert _written(buf) == {(4, 4)}
+    buf[:] = 0
+    lib.rc_line(w, h, p, 10, 10, 2, 2, c)  # 45 degrees backwards
+    assert _written(buf) == {(i, i) for i in range(2, 11)}
