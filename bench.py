@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--tile", default="32x32",
                     help="raster tile WxH, or 'auto' to time the candidates first and keep "
                          "the fastest")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the throughput lines of configs 3-5")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run k_copy over this many MiB (known-size kernel that the "
                          "HBM byte counters are calibrated against)")
@@ -62,6 +64,57 @@ def measured_traffic(kernel, cfg, frames_per_launch):
         if line.startswith("traffic_bytes_per_launch ") and want in line:
             return int(line.rsplit("=", 1)[1])
     return None
+
+
+def other_configs(ctx, scenes):
+    """Throughput of configs 3-5 on the same context: config 3 (64 instances,
+    ~780k triangles, 1920x1080) drawn one frame per call; config 4 (16 frames of
+    1k-500k triangles at four resolutions) as one batch; config 5 (8 requests of
+    1-8 objects over 32 resident meshes, ~8M triangles) as one batch."""
+    import numpy as np
+    out = {}
+
+    def run(name, frames, handles, reps, batch):
+        bufs = [ctx.frame_buffers(s.frame.width, s.frame.height) for s in frames]
+        ptrs = [(b[0].ptr, b[1].ptr) for b in bufs]
+        tris = sum(s.num_faces for s in frames)
+        if batch:
+            prepared = [ctx.prepare_batch(frames, handles, ptrs)]
+        else:
+            prepared = [ctx.prepare_batch([s], [h], [p]) for s, h, p in zip(frames, handles, ptrs)]
+        for _ in range(2):  # warm-up (grows tile-list capacity if needed)
+            for pb in prepared:
+                ctx.submit_batch(pb)
+            try:
+                ctx.synchronize()
+            except Exception:
+                pass
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for pb in prepared:
+                ctx.submit_batch(pb)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"mtris_s": round(tris * reps / dt / 1e6, 1),
+                     "frames_per_s": round(len(frames) * reps / dt, 1),
+                     "triangles_per_frame_avg": int(tris / len(frames)),
+                     "form": "batched" if batch else "plain"}
+        for b in bufs:
+            b[0].free()
+            b[1].free()
+
+    c3 = scenes.config3()
+    th = [ctx.upload_texture(t) for t in c3.textures]
+    mh = [ctx.upload_mesh(m, th) for m in c3.meshes]
+    run("config3_1080p_64_instances", [c3], [mh], 100, False)
+    c4 = scenes.config4_frames(16)
+    h4 = [ctx.upload_scene(s)[0] for s in c4]
+    run("config4_16_mixed_frames", c4, h4, 40, True)
+    c5 = scenes.config5(8)
+    th = [ctx.upload_texture(t) for t in c5.textures]
+    mh = [ctx.upload_mesh(m, th) for m in c5.meshes]
+    run("config5_8_requests_8M_resident", c5.requests, [mh] * len(c5.requests), 20, True)
+    return out
 
 
 def pct(xs, p):
@@ -189,6 +242,10 @@ def main():
         src.free()
         dst.free()
 
+    extra = None
+    if not args.no_extra and not args.profile:
+        extra = other_configs(ctx, scenes)
+
     lat_dev, lat_host = [], []
     if not args.profile:
         for k in range(100):
@@ -304,6 +361,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_forms": cpu_forms,
         "tile_sweep_fps": tile_sweep or None,
+        "other_configs": extra,
     }
     print(json.dumps(out))
     ctx.close()
