@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--tile", default="32x32",
                     help="raster tile WxH, or 'auto' to time the candidates first and keep "
                          "the fastest")
+    ap.add_argument("--tile-split", type=int, default=0,
+                    help="tile-list entries per raster workgroup before a tile is split "
+                         "(0 keeps the library default)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the throughput lines of configs 3-5")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -174,7 +177,7 @@ def main():
             ctx.submit_batch(singles[k % n_variants])
 
     def set_tile(tw, th):
-        ctx.set_config(tile_w=tw, tile_h=th)
+        ctx.set_config(tile_w=tw, tile_h=th, tile_split=args.tile_split)
 
     tile_sweep = {}
     if args.tile == "auto":
@@ -334,7 +337,8 @@ def main():
         "config": {"workload": "headline frame: 250x400 lat-long displaced sphere, 200,000 "
                                "triangles, 16x16 RGBA8 texture, 1280x720, camera r=1.2",
                    "frames_per_batch": B, "triangles_per_frame": faces,
-                   "tile": [ctx.config().tile_w, ctx.config().tile_h]},
+                   "tile": [ctx.config().tile_w, ctx.config().tile_h],
+                   "tile_split": ctx.config().tile_split},
         "shaded_mpix_s": round(shaded * main_fps / 1e6, 3),
         "covered_mfrag_s": round(covered * main_fps / 1e6, 3),
         "frames_per_s": {"batched": round(frames_s_b, 2), "plain": round(frames_s_p, 2),

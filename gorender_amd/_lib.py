@@ -67,7 +67,7 @@ class gr_frame(Structure):
 class gr_stats(Structure):
     _fields_ = [("tpf", c_int64), ("triangles", c_int64), ("fragments_covered", c_int64),
                 ("pixels_shaded", c_int64), ("faces_front", c_int64),
-                ("clip_overflow", c_int64), ("refs", c_int64), ("reserved", c_int64)]
+                ("clip_overflow", c_int64), ("refs", c_int64), ("tiles_split", c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
@@ -80,7 +80,7 @@ class gr_batch_frame(Structure):
 
 class gr_config(Structure):
     _fields_ = [("tile_w", c_int32), ("tile_h", c_int32), ("refs_capacity", c_int64),
-                ("raster_waves", c_int32), ("reserved", c_int32)]
+                ("raster_waves", c_int32), ("tile_split", c_int32)]
 
 
 # name -> (restype, argtypes); every symbol of include/gorender_hip.h

@@ -85,6 +85,8 @@ struct gr_context {
     // batch: frames | objects | stats | counters | frame_overflow,
     // uploaded with one copy per draw (the counters as zeros).
     DevBuf batch, rrec, srec, wrec, tile_counts, tiles, wgcount, refs, out_rgba, out_depth, ctrl;
+    DevBuf items;  // raster work items (later parts of split tiles)
+    DevBuf slabs;  // depth keys of split tiles; zero between draws
     DevBuf okey;  // overlay keys of the frames that draw overlays; zero between draws
     DevBuf orec;  // overlay records per slot (batches with overlay frames)
     Staging ring[4];
@@ -93,7 +95,7 @@ struct gr_context {
     int32_t last_frames = 0;
     const unsigned long long* last_stats = nullptr;  // device stats of the last draw
 
-    gr_config cfg{32, 32, 1 << 22, 4, 0};
+    gr_config cfg{32, 32, 1 << 22, 4, 512};
 
     bool timing = false;
     std::vector<TimedLaunch> timed;
@@ -262,6 +264,35 @@ float go_max(float a, float b) {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Whether a mesh has at least one face per 4 pixels of its bounding box
+// projected on screen (false when the box crosses the plane W = 0 or a
+// projection is not finite). k_face counts such meshes' small tile rectangles
+// through LDS; the estimate only steers that choice, never the result.
+bool dense_on_screen(const float* mvp, const float bbox[8][4], int F, int W, int H) {
+    float x0 = 1.f, x1 = -1.f, y0 = 1.f, y1 = -1.f;
+    int neg = 0;
+    for (int k = 0; k < 8; k++) {
+        const float* c = bbox[k];
+        const float X = mvp[0] * c[0] + mvp[1] * c[1] + mvp[2] * c[2] + mvp[3] * c[3];
+        const float Y = mvp[4] * c[0] + mvp[5] * c[1] + mvp[6] * c[2] + mvp[7] * c[3];
+        const float Wc = mvp[12] * c[0] + mvp[13] * c[1] + mvp[14] * c[2] + mvp[15] * c[3];
+        neg += Wc < 0.f;
+        if (!std::isfinite(X / Wc) || !std::isfinite(Y / Wc)) return false;
+        x0 = std::min(x0, X / Wc);
+        x1 = std::max(x1, X / Wc);
+        y0 = std::min(y0, Y / Wc);
+        y1 = std::max(y1, Y / Wc);
+    }
+    if (neg != 0 && neg != 8) return false;
+    x0 = std::max(x0, -1.f), x1 = std::min(x1, 1.f);
+    y0 = std::max(y0, -1.f), y1 = std::min(y1, 1.f);
+    const double area = std::max(0.0, (double)(x1 - x0) * W / 2) * std::max(0.0, (double)(y1 - y0) * H / 2);
+    return (double)F * 4.0 >= std::max(area, 1.0);
+}
+
+// Device memory for the depth keys of split tiles.
+constexpr unsigned long long kSlabBudget = 64ull << 20;
+
 // The whole pipeline for a batch of frames. Objects must be laid out frame by
 // frame; outputs are device pointers.
 gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
@@ -334,6 +365,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             if (o.gouraud && !mr.ni_valid)
                 return fail(GR_ERR_INDEX_OUT_OF_RANGE,
                             "normal index out of range under Gouraud shading");
+            o.dense = dense_on_screen(ob.mvp, mr.dm.bbox, mr.dm.F, fr.width, fr.height) ? 1 : 0;
             o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
             o.face_block = fblocks;
             o.seq_block = seq;
@@ -392,7 +424,22 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         if (c->tile_counts.p != old)
             GR_HIP(hipMemsetAsync(c->tile_counts.p, 0, c->tile_counts.cap, c->stream));
     }
-    if ((st = ensure(c, c->tiles, 2 * (size_t)tiles * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, c->tiles, 3 * (size_t)tiles * sizeof(uint32_t)))) return st;
+    // Split tiles: a tile is split only when its list is longer than part_refs,
+    // so the items of a batch number at most refs_cap / part_refs and the split
+    // tiles at most refs_cap / (part_refs + 1); slabs are further capped by a
+    // memory budget (a tile without a slab is rasterised whole).
+    const uint32_t part_refs = (uint32_t)c->cfg.tile_split;
+    const uint32_t max_items = (uint32_t)std::min<unsigned long long>(refs_cap / part_refs + 1, 0xFFFFFFF0ull);
+    const size_t slab_bytes = (size_t)TW * TH * 8;
+    const uint32_t max_slabs = (uint32_t)std::max<unsigned long long>(
+        1, std::min<unsigned long long>(refs_cap / (part_refs + 1ull) + 1, kSlabBudget / slab_bytes));
+    if ((st = ensure(c, c->items, (size_t)max_items * sizeof(gr::RasterItem)))) return st;
+    if (c->slabs.cap < (size_t)max_slabs * slab_bytes) {
+        void* old = c->slabs.p;
+        if ((st = ensure(c, c->slabs, (size_t)max_slabs * slab_bytes))) return st;
+        if (c->slabs.p != old) GR_HIP(hipMemsetAsync(c->slabs.p, 0, c->slabs.cap, c->stream));
+    }
     if ((st = ensure(c, c->wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
     if ((st = ensure(c, c->refs, refs_cap * sizeof(uint32_t)))) return st;
     if (!c->ctrl.p) {
@@ -443,6 +490,12 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     p.tile_counts = (uint32_t*)c->tile_counts.p;
     p.tile_offsets = (uint32_t*)c->tiles.p;
     p.tile_cursor = (uint32_t*)c->tiles.p + tiles;
+    p.tile_slab = (uint32_t*)c->tiles.p + 2 * (size_t)tiles;
+    p.items = (gr::RasterItem*)c->items.p;
+    p.max_items = max_items;
+    p.slabs = (unsigned long long*)c->slabs.p;
+    p.max_slabs = max_slabs;
+    p.part_refs = part_refs;
     p.frame_overflow = (uint32_t*)(bb + off_over);
     p.wg_count = (uint32_t*)c->wgcount.p;
     p.refs = (uint32_t*)c->refs.p;
@@ -458,7 +511,8 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     timed_launch(c, "k_bin_fill", [&] { gr::launch_bin_fill(p, c->stream); });
     if (okey_words) timed_launch(c, "k_overlay", [&] { gr::launch_overlay(p, c->stream); });
     bool ok = true;
-    timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
+    timed_launch(c, "k_raster_parts", [&] { ok = gr::launch_raster_parts(p, c->stream); });
+    if (ok) timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
     hipError_t le = hipGetLastError();
     if (!ok || le != hipSuccess) {
         // Leave the tile counters zeroed for the next draw.
@@ -488,7 +542,7 @@ void to_gr_stats(const unsigned long long* q, gr_stats* s) {
     s->faces_front = (int64_t)q[gr::kStatFront];
     s->clip_overflow = (int64_t)q[gr::kStatClipOverflow];
     s->refs = (int64_t)q[gr::kStatRefs];
-    s->reserved = 0;
+    s->tiles_split = (int64_t)q[gr::kStatSplit];
 }
 
 // Wait, read and clear the sticky overflow words.
@@ -586,7 +640,7 @@ void gr_destroy(gr_context* c) {
         if (t.texels) (void)hipFree(t.texels);
     DevBuf* bufs[] = {&c->d_textures, &c->batch, &c->rrec, &c->srec, &c->wrec,
                       &c->tile_counts, &c->tiles, &c->wgcount, &c->refs, &c->out_rgba, &c->okey, &c->orec,
-                      &c->out_depth, &c->ctrl};
+                      &c->out_depth, &c->ctrl, &c->items, &c->slabs};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& s : c->ring) {
@@ -610,10 +664,13 @@ gr_status gr_set_config(gr_context* c, const gr_config* cfg) {
     if (cfg->tile_h) n.tile_h = cfg->tile_h;
     if (cfg->refs_capacity) n.refs_capacity = cfg->refs_capacity;
     if (cfg->raster_waves) n.raster_waves = cfg->raster_waves;
+    if (cfg->tile_split) n.tile_split = cfg->tile_split;
     if (!valid_tile(n.tile_w) || !valid_tile(n.tile_h))
         return fail(GR_ERR_INVALID_ARGUMENT, "tile sides must be 8, 16, 32 or 64");
     if (n.raster_waves != 4) return fail(GR_ERR_INVALID_ARGUMENT, "raster_waves must be 4");
     if (n.refs_capacity < 1024) return fail(GR_ERR_INVALID_ARGUMENT, "refs_capacity too small");
+    if (n.tile_split < 64 || n.tile_split > (1 << 24))
+        return fail(GR_ERR_INVALID_ARGUMENT, "tile_split must be in [64, 2^24]");
     c->cfg = n;
     return GR_OK;
 }

@@ -62,7 +62,7 @@ struct DevObject {
     int32_t face_block;     // first k_face block (batch-global)
     int32_t face_blocks;    // number of k_face blocks
     int32_t seq_block;      // first face block index within the frame (draw order)
-    int32_t pad;
+    int32_t dense;          // >= 1 face per 4 pixels of the projected bounding box
 };
 
 struct DevTexture {
@@ -144,14 +144,25 @@ enum : int32_t {
     kStatFront,
     kStatClipOverflow,
     kStatRefs,
-    kStatReserved,
+    kStatSplit,  // tiles whose list was split across raster workgroups
     kNumStats
 };
 
 // Per-batch counters, uploaded as zeros with the descriptors of every draw.
 struct DevBatchCounters {
     unsigned long long refs_used;  // tile-list entries allocated so far
-    unsigned long long pad;
+    uint32_t n_items;              // raster work items (later parts of split tiles)
+    uint32_t next_item;            // next work item to take
+    uint32_t n_slabs;              // slabs handed to split tiles
+    uint32_t pad[3];
+};
+
+// Tile not split (no slab); also the end of the raster work queue.
+constexpr uint32_t kNoSlab = 0xFFFFFFFFu;
+
+// Raster work item: a later part (>= 1) of a split tile.
+struct RasterItem {
+    uint32_t tile, part;
 };
 
 // Sticky control words (cleared by the host when it reads them): set when a

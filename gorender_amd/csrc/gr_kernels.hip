@@ -18,6 +18,7 @@
 // uses -ffp-contract=off and correctly rounded division/sqrt.
 #pragma clang fp contract(off)
 
+#include <algorithm>
 #include <cstddef>
 
 #include "gr_device.h"
@@ -464,11 +465,58 @@ __device__ void draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t 
         for (int j = 0; j < 3; j++) ov_rect(fr, wsub(ix[j], 1), wsub(iy[j], 1), code | 1u);
 }
 
+// Tile references of a triangle's tile rectangle, counted for the list scan.
+// Dense meshes put thousands of small triangles into the same tile; counted one
+// global atomic each, they serialise on that counter. One-tile rectangles, and
+// for dense meshes (at least one face per 4 pixels of the projected bounding
+// box) rectangles of up to 2x2 tiles, are therefore counted in a per-workgroup
+// LDS table of (tile, count) entries, flushed with one global atomic per entry
+// when the workgroup ends (a tile that finds its entry held by another tile is
+// counted in global memory). Other rectangles, whose tiles see few references
+// each, add one global atomic per tile without the table's round trips.
+constexpr int kTileTable = 256;
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+struct TileTally {
+    lds_u32* tag;
+    lds_u32* cnt;
+    uint32_t* global;
+    bool dense;
+};
+
+__device__ __forceinline__ void tally_tile(const TileTally& tally, uint32_t t) {
+    const uint32_t h = t & (uint32_t)(kTileTable - 1);
+    uint32_t old = kNoTile;  // the entry's previous tag when the exchange fails
+    __hip_atomic_compare_exchange_strong(tally.tag + h, &old, t, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == kNoTile || old == t)
+        __hip_atomic_fetch_add(tally.cnt + h, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        atomicAdd(&tally.global[t], 1u);
+}
+
+__device__ __forceinline__ void count_tiles(const DevFrame& fr, const TileTally& tally, int tx0,
+                                            int tx1, int ty0, int ty1) {
+    if (tx0 == tx1 && ty0 == ty1) {
+        tally_tile(tally, (uint32_t)(fr.tile_begin + ty0 * fr.tiles_x + tx0));
+        return;
+    }
+    const bool small = tally.dense && tx1 - tx0 <= 1 && ty1 - ty0 <= 1;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            const uint32_t t = (uint32_t)(fr.tile_begin + ty * fr.tiles_x + tx);
+            if (small)
+                tally_tile(tally, t);
+            else
+                atomicAdd(&tally.global[t], 1u);
+        }
+}
+
 // Writes the set-up of one post-clip triangle into its slot (renderer.go:361-394
 // then the per-triangle part of rasterizer.go:98-137). Returns the TPF
 // references in the low 16 bits and 1 << 16 when the triangle was binned.
 __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, const CV& c,
-                                  int tex, uint32_t slot, uint32_t* tile_counts, int twl,
+                                  int tex, uint32_t slot, const TileTally& tally, int twl,
                                   int thl) {
     uint32_t tpf = 0, tris = 0;
     const CV vv[3] = {a, b, c};
@@ -632,14 +680,9 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
         }
     }
     fr.rrec[slot] = rr;
-    // Tile references of the triangle, counted for the list scan.
-    if (rr.bx0 <= rr.bx1) {
-        const int tx0 = (int)rr.bx0 >> twl, tx1 = (int)rr.bx1 >> twl;
-        const int ty0 = (int)rr.by0 >> thl, ty1 = (int)rr.by1 >> thl;
-        for (int ty = ty0; ty <= ty1; ty++)
-            for (int tx = tx0; tx <= tx1; tx++)
-                atomicAdd(&tile_counts[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
-    }
+    if (rr.bx0 <= rr.bx1)
+        count_tiles(fr, tally, (int)rr.bx0 >> twl, (int)rr.bx1 >> twl, (int)rr.by0 >> thl,
+                    (int)rr.by1 >> thl);
     return tpf | (tris << 16);
 }
 
@@ -818,12 +861,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     __shared__ CV wpoly[4][2][kMaxClip];
     __shared__ CV lds_stage[4][kStage][kMaxClip];  // clipped polygons of general faces
     __shared__ int lds_stage_k[4][kStage], lds_stage_tex[4][kStage], lds_stage_slot[4][kStage];
+    __shared__ uint32_t lds_tile_tag[kTileTable], lds_tile_cnt[kTileTable];
 
     int bf, blocal;
     if (!xcd_unit(xcd_face, nframes, (int)blockIdx.x, bf, blocal)) return;
+    // The barrier of the survivor compaction orders this before any count.
+    for (int i = threadIdx.x; i < kTileTable; i += 256) {
+        lds_tile_tag[i] = kNoTile;
+        lds_tile_cnt[i] = 0u;
+    }
     const int b = frames[bf].face_block_begin + blocal;
     const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
     const DevObject& o = objs[oi];
+    const TileTally tally{(lds_u32*)lds_tile_tag, (lds_u32*)lds_tile_cnt, tile_counts,
+                          o.dense != 0};
     const DevMesh& m = o.m;
     const DevFrame& fr = frames[o.frame];
     const int wg_in_obj = b - o.face_block;
@@ -964,11 +1015,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int off = block_exclusive_scan(n, lds_scan, total);
     const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
     const uint32_t slot0 = wg_slot0 + (uint32_t)off;
-    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0, tile_counts, twl, thl));
+    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0, tally, twl, thl));
     if (mode == 3) {
-        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tile_counts, twl, thl));
+        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tally, twl, thl));
         if (nq == 4)
-            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tile_counts, twl,
+            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl,
                                        thl));
     }
 
@@ -1004,7 +1055,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             const CV* P = lds_stage[wave][r];
             if (t < k - 2)
                 add_emit(st, emit_triangle(fr, P[0], P[t + 1], P[t + 2], lds_stage_tex[wave][r],
-                                           slot, tile_counts, twl, thl));
+                                           slot, tally, twl, thl));
             else
                 empty_record(fr, slot);
         }
@@ -1031,6 +1082,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                lds_st[2][threadIdx.x] + lds_st[3][threadIdx.x];
         const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
         if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
+    }
+    // The barrier above also ends every count: flush the table.
+    for (int i = threadIdx.x; i < kTileTable; i += 256) {
+        const uint32_t v = lds_tile_cnt[i];
+        if (v) atomicAdd(&tile_counts[lds_tile_tag[i]], v);
     }
 }
 
@@ -1069,7 +1125,11 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
                                                    unsigned long long* __restrict__ stats,
                                                    DevBatchCounters* __restrict__ counters,
                                                    DevControl* __restrict__ ctrl,
-                                                   unsigned long long refs_capacity) {
+                                                   unsigned long long refs_capacity,
+                                                   uint32_t* __restrict__ tile_slab,
+                                                   RasterItem* __restrict__ items,
+                                                   uint32_t max_items, uint32_t max_slabs,
+                                                   uint32_t part_refs) {
     __shared__ unsigned long long wsum[16];
     __shared__ unsigned long long base_s;
     const int f = blockIdx.x;
@@ -1119,6 +1179,25 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
             tile_offsets[fr.tile_begin + i] = (uint32_t)offv;
             tile_cursor[fr.tile_begin + i] = (uint32_t)offv;
             if (over) cnt[i] = 0u;  // overflowed frame: raster draws background only
+            if (!over && v > part_refs) {
+                // Split the list into parts of part_refs entries: part 0 stays
+                // with the tile's workgroup, the others become work items. With
+                // no slab left the tile stays whole (slower, same result).
+                const uint32_t np = (v - 1u) / part_refs + 1u;
+                uint32_t slab = atomicAdd(&counters->n_slabs, 1u);
+                if (slab < max_slabs) {
+                    atomicAdd(&stats[f * kNumStats + kStatSplit], 1ull);
+                    // Items of all split tiles number at most refs / part_refs,
+                    // and max_items is sized for the whole tile-list capacity.
+                    const uint32_t ib = atomicAdd(&counters->n_items, np - 1u);
+                    for (uint32_t k = 1; k < np; k++)
+                        if (ib + k - 1u < max_items)
+                            items[ib + k - 1u] = RasterItem{(uint32_t)(fr.tile_begin + i), k};
+                } else {
+                    slab = kNoSlab;
+                }
+                tile_slab[fr.tile_begin + i] = slab;
+            }
         }
         run += ctot;
     }
@@ -1352,40 +1431,23 @@ constexpr int raster_waves(int npx) {
                : (160 * 1024) / (npx * 8 + kRasterLdsFixed);
 }
 
+// Resolves n tile-list entries (slots) into the tile's depth keys: the
+// visibility part of rasterizer.go:139-156 for every listed triangle. Returns
+// this lane's share of the covered-fragment count (summed over the wave).
 template <int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
-void k_raster(const DevFrame* __restrict__ frames, int nframes,
-                                                uint32_t* __restrict__ tile_counts,
-                                                const uint32_t* __restrict__ tile_offsets,
-                                                const uint32_t* __restrict__ refs,
-                                                const DevTexture* __restrict__ textures,
-                                                unsigned long long* __restrict__ stats) {
-    constexpr int NPX = TW * TH;
+__device__ __forceinline__ unsigned long long raster_list(
+    const DevFrame& fr, int X0, int Y0, const uint32_t* __restrict__ list, uint32_t cnt,
+    unsigned long long* key, float4 (*ldiv)[2], uint32_t* lslot, uint32_t (*q)[128], int lane,
+    int wave) {
     constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
-    __shared__ unsigned long long key[NPX];
-    __shared__ float4 ldiv[CHUNK][2];  // divisors and reciprocals of the staged records
-    __shared__ uint32_t lslot[CHUNK];
-    __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
-    __shared__ unsigned long long red[4][2];
-    const int g = blockIdx.x;
-    const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
-    const DevFrame& fr = frames[fi];
-    const int t = g - fr.tile_begin;
-    const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
-    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
-    uint32_t(*const q)[128] = qbuf[wave];
-
-    for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
-
-    const uint32_t cnt = tile_counts[g], off = tile_offsets[g];
     unsigned long long covered_lane = 0;  // wide path, per lane
     uint32_t covered_wave = 0;            // int32 path, wave-uniform
     for (uint32_t base = 0; base < cnt; base += CHUNK) {
         const uint32_t n = min(cnt - base, (uint32_t)CHUNK);
-        // Stage the chunk: slots (one coalesced read), then the raster records
-        // as 16-byte pieces.
+        // Stage the chunk: slots (one coalesced read), then the divisors and
+        // reciprocals of the raster records as 16-byte pieces.
         __syncthreads();
-        if (threadIdx.x < n) lslot[threadIdx.x] = refs[off + base + threadIdx.x];
+        if (threadIdx.x < n) lslot[threadIdx.x] = list[base + threadIdx.x];
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < 2 * n; p += 256) {
             const uint32_t ri = p >> 1, pi = p & 1u;
@@ -1496,16 +1558,21 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
             qhead += cnt_q;
         }
     }
-    __syncthreads();
-    // Tile counts are zero between draws: the face stage of the next draw
-    // accumulates into them again.
-    if (threadIdx.x == 0 && cnt) tile_counts[g] = 0u;
+    return covered_lane + (lane == 0 ? (unsigned long long)covered_wave : 0ull);
+}
 
-    // Deferred shading of the winning fragment of each pixel, one write per pixel.
-    // Background colours are read as values and the dot test is evaluated
-    // without short-circuit branches: hipcc 7.2 (clang 22) turned the
-    // short-circuit form into a select of field addresses that dropped the
-    // y % step condition when the loop runs once per thread.
+// Deferred shading of the winning fragment of each pixel of the tile, one
+// write per pixel. Returns this thread's count of shaded pixels.
+//
+// Background colours are read as values and the dot test is evaluated
+// without short-circuit branches: hipcc 7.2 (clang 22) turned the
+// short-circuit form into a select of field addresses that dropped the
+// y % step condition when the loop runs once per thread.
+template <int TW, int TH>
+__device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int X0, int Y0,
+                                                         const unsigned long long* key,
+                                                         const DevTexture* __restrict__ textures) {
+    constexpr int NPX = TW * TH;
     const uint32_t dot_color = fr.dot_color, clear_color = fr.clear_color;
     const int step = fr.dot_step;
     const int step1 = step >= 1 ? step : 1;
@@ -1580,11 +1647,141 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
         ((uint32_t*)fr.rgba)[idx] = color;
         fr.depth[idx] = depth;
     }
+    return shaded;
+}
 
-    const unsigned long long covered = wave_sum_u64(covered_lane) + covered_wave;
-    shaded = wave_sum_u64(shaded);
+// A tile whose list holds more than part_refs entries, and that k_bin_scan gave
+// a slab, is split into parts of part_refs entries. Parts 1.. are work items
+// that k_raster_parts resolves first, any workgroup taking the next item from
+// the batch queue: each resolves its entries into LDS keys and merges them into
+// the tile's slab in HBM with a 64-bit max (the (depth, sequence) resolve does
+// not depend on order). k_raster then resolves part 0 of every tile, folds in
+// the slab of a split tile (and clears it) and shades.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
+void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
+                    const uint32_t* __restrict__ tile_counts,
+                    const uint32_t* __restrict__ tile_offsets,
+                    const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
+                    unsigned long long* __restrict__ stats,
+                    DevBatchCounters* __restrict__ counters, const RasterItem* __restrict__ items,
+                    uint32_t max_items, unsigned long long* __restrict__ slabs,
+                    uint32_t part_refs) {
+    constexpr int NPX = TW * TH;
+    __shared__ unsigned long long key[NPX];
+    __shared__ float4 ldiv[kRasterChunk][2];
+    __shared__ uint32_t lslot[kRasterChunk];
+    __shared__ uint32_t qbuf[4][3][128];
+    __shared__ unsigned long long red[4];
+    __shared__ RasterItem litem;
+    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
+    // Written by k_bin_scan (an earlier kernel).
+    const uint32_t n_items = min(counters->n_items, max_items);
+    if (n_items == 0u) return;
+    // Every branch around the barriers of this loop is wave-uniform (scalar):
+    // thread-level branches here (a thread-0 fetch, an exit on a value read
+    // from LDS) let the compiler restructure the loop so that thread 0 leaves
+    // it while the other threads go round the barriers again, which hangs.
+    for (;;) {
+        if (wave == 0) {
+            // Lane 0 fetches the next item; the wave publishes it.
+            uint32_t it = kNoSlab;
+            if (lane == 0) {
+                it = __hip_atomic_load(&counters->next_item, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                if (it < n_items)
+                    it = __hip_atomic_fetch_add(&counters->next_item, 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            }
+            it = (uint32_t)sgpr((int)it);
+            const RasterItem w = it < n_items ? items[it] : RasterItem{kNoSlab, 0u};
+            if (lane == 0) litem = w;
+        }
+        __syncthreads();
+        const uint32_t wtile = (uint32_t)sgpr((int)litem.tile);
+        const uint32_t wpart = (uint32_t)sgpr((int)litem.part);
+        if (wtile == kNoSlab) return;
+        const int tile = (int)wtile;
+        const int fi = last_le(nframes, tile, [&](int i) { return frames[i].tile_begin; });
+        const DevFrame& fr = frames[fi];
+        const int t = tile - fr.tile_begin;
+        const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
+        const uint32_t cnt = tile_counts[tile], beg = wpart * part_refs;
+        for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
+        // raster_list's first barrier orders the initialisation (and the read of
+        // litem) before what follows.
+        unsigned long long covered =
+            raster_list<TW, TH>(fr, X0, Y0, refs + tile_offsets[tile] + beg,
+                                min(cnt - beg, part_refs), key, ldiv, lslot, qbuf[wave], lane,
+                                wave);
+        __syncthreads();
+        unsigned long long* slab = slabs + (size_t)tile_slab[tile] * NPX;
+        for (int i = threadIdx.x; i < NPX; i += 256) {
+            const unsigned long long k = key[i];
+            if ((uint32_t)k != 0u) atomicMax(&slab[i], k);
+        }
+        covered = wave_sum_u64(covered);
+        if (lane == 0) red[wave] = covered;
+        __syncthreads();
+        if (wave == 0) {
+            const unsigned long long c = red[0] + red[1] + red[2] + red[3];
+            if (lane == 0 && c) atomicAdd(&stats[fi * kNumStats + kStatCovered], c);
+        }
+        // litem and red are read before the next item overwrites them.
+        __syncthreads();
+    }
+}
+
+// One workgroup per screen tile: part 0 of its list (all of it when the tile is
+// not split), then the deferred shading.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
+void k_raster(const DevFrame* __restrict__ frames, int nframes,
+              uint32_t* __restrict__ tile_counts, const uint32_t* __restrict__ tile_offsets,
+              const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
+              const DevTexture* __restrict__ textures, unsigned long long* __restrict__ stats,
+              unsigned long long* __restrict__ slabs, uint32_t part_refs) {
+    constexpr int NPX = TW * TH;
+    __shared__ unsigned long long key[NPX];
+    __shared__ float4 ldiv[kRasterChunk][2];  // divisors and reciprocals of the staged records
+    __shared__ uint32_t lslot[kRasterChunk];
+    __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
+    __shared__ unsigned long long red[4][2];
+    const int g = blockIdx.x;
+    const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
+    const DevFrame& fr = frames[fi];
+    const int t = g - fr.tile_begin;
+    const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
+    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
+
+    for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
+
+    const uint32_t cnt = tile_counts[g], off = tile_offsets[g];
+    const uint32_t slab_i = cnt > part_refs ? tile_slab[g] : kNoSlab;
+    // raster_list's first barrier orders the initialisation before its use.
+    const unsigned long long covered = raster_list<TW, TH>(
+        fr, X0, Y0, refs + off, slab_i != kNoSlab ? part_refs : cnt, key, ldiv, lslot, qbuf[wave],
+        lane, wave);
+    __syncthreads();
+    if (slab_i != kNoSlab) {
+        // Fold in the later parts (merged by k_raster_parts; 0 = no fragment)
+        // and leave the slab cleared for the next draw.
+        unsigned long long* slab = slabs + (size_t)slab_i * NPX;
+        for (int i = threadIdx.x; i < NPX; i += 256) {
+            const unsigned long long k = slab[i];
+            if (k > key[i]) key[i] = k;
+            slab[i] = 0ull;
+        }
+        __syncthreads();
+    }
+    // Tile counts are zero between draws: the face stage of the next draw
+    // accumulates into them again.
+    if (threadIdx.x == 0 && cnt) tile_counts[g] = 0u;
+
+    const unsigned long long shaded = wave_sum_u64(shade_tile<TW, TH>(fr, X0, Y0, key, textures));
+    const unsigned long long cov = wave_sum_u64(covered);
     if (lane == 0) {
-        red[wave][0] = covered;
+        red[wave][0] = cov;
         red[wave][1] = shaded;
     }
     __syncthreads();
@@ -1656,7 +1853,8 @@ void launch_overlay(const LaunchParams& p, hipStream_t s) {
 void launch_bin_scan(const LaunchParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_scan, dim3(p.nframes), dim3(1024), 0, s, p.frames, p.tile_counts,
                        p.tile_offsets, p.tile_cursor, p.frame_overflow, p.stats, p.counters,
-                       p.ctrl, p.refs_capacity);
+                       p.ctrl, p.refs_capacity, p.tile_slab, p.items, p.max_items, p.max_slabs,
+                       p.part_refs);
 }
 
 void launch_bin_fill(const LaunchParams& p, hipStream_t s) {
@@ -1768,16 +1966,28 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
 }
 
 template <int TW, int TH>
-static void raster_t(const LaunchParams& p, hipStream_t s) {
+static void raster_t(const LaunchParams& p, hipStream_t s, bool parts) {
+    if (parts) {
+        // Workgroups of the parts kernel: enough to fill every CU at the
+        // raster's occupancy; with no work items they return at once.
+        const uint32_t grid =
+            std::min<uint32_t>(p.max_items, 256u * (uint32_t)raster_waves(TW * TH));
+        if (grid)
+            hipLaunchKernelGGL((k_raster_parts<TW, TH>), dim3(grid), dim3(256), 0, s, p.frames,
+                               p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs,
+                               p.stats, p.counters, p.items, p.max_items, p.slabs, p.part_refs);
+        return;
+    }
     hipLaunchKernelGGL((k_raster<TW, TH>), dim3(p.total_tiles), dim3(256), 0, s, p.frames,
-                       p.nframes, p.tile_counts, p.tile_offsets, p.refs, p.textures, p.stats);
+                       p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs, p.textures,
+                       p.stats, p.slabs, p.part_refs);
 }
 
-bool launch_raster(const LaunchParams& p, hipStream_t s) {
+static bool raster_dispatch(const LaunchParams& p, hipStream_t s, bool parts) {
     if (p.total_tiles == 0) return true;
 #define GR_RASTER_CASE(W, H)                          \
     if (p.tw_log2 == W && p.th_log2 == H) {           \
-        raster_t<(1 << W), (1 << H)>(p, s);           \
+        raster_t<(1 << W), (1 << H)>(p, s, parts);    \
         return true;                                  \
     }
     GR_RASTER_CASE(3, 3) GR_RASTER_CASE(3, 4) GR_RASTER_CASE(3, 5) GR_RASTER_CASE(3, 6)
@@ -1787,5 +1997,8 @@ bool launch_raster(const LaunchParams& p, hipStream_t s) {
 #undef GR_RASTER_CASE
     return false;
 }
+
+bool launch_raster_parts(const LaunchParams& p, hipStream_t s) { return raster_dispatch(p, s, true); }
+bool launch_raster(const LaunchParams& p, hipStream_t s) { return raster_dispatch(p, s, false); }
 
 }  // namespace gr

@@ -22,6 +22,12 @@ struct LaunchParams {
     uint32_t* tile_counts;   // [total_tiles]; zero between draws (k_raster re-zeroes)
     uint32_t* tile_offsets;  // [total_tiles]
     uint32_t* tile_cursor;   // [total_tiles]
+    uint32_t* tile_slab;     // [total_tiles]: slab of a split tile or kNoSlab
+    RasterItem* items;       // [max_items]: later parts of split tiles
+    uint32_t max_items;
+    unsigned long long* slabs;  // [max_slabs][tile pixels] keys; zero between draws
+    uint32_t max_slabs;
+    uint32_t part_refs;      // tile-list entries per raster part
     uint32_t* frame_overflow;// [nframes]
     uint32_t* wg_count;      // [face_blocks]
     uint32_t* refs;          // [refs_capacity]
@@ -47,6 +53,7 @@ void launch_check_div(uint64_t n, uint64_t seed, unsigned long long* mismatches,
 void launch_bin_scan(const LaunchParams& p, hipStream_t s);
 void launch_overlay(const LaunchParams& p, hipStream_t s);
 void launch_bin_fill(const LaunchParams& p, hipStream_t s);
+bool launch_raster_parts(const LaunchParams& p, hipStream_t s);
 bool launch_raster(const LaunchParams& p, hipStream_t s);
 
 }  // namespace gr

@@ -64,8 +64,10 @@ class Context:
             pass
 
     # ---- configuration -------------------------------------------------
-    def set_config(self, tile_w: int = 0, tile_h: int = 0, refs_capacity: int = 0) -> None:
-        c = _lib.gr_config(tile_w, tile_h, refs_capacity, 0, 0)
+    def set_config(self, tile_w: int = 0, tile_h: int = 0, refs_capacity: int = 0,
+                   tile_split: int = 0) -> None:
+        """Zero keeps the current value (gr_set_config)."""
+        c = _lib.gr_config(tile_w, tile_h, refs_capacity, 0, tile_split)
         _lib.check(self.lib.gr_set_config(self.handle, ctypes.byref(c)))
 
     def config(self) -> _lib.gr_config:

@@ -133,7 +133,7 @@ typedef struct {
     int64_t faces_front;       /* faces passing the backface test */
     int64_t clip_overflow;     /* faces dropped by GR_ERR_CLIP_OVERFLOW */
     int64_t refs;              /* tile-list entries used */
-    int64_t reserved;
+    int64_t tiles_split;       /* tiles whose list several raster workgroups shared */
 } gr_stats;
 
 /* One frame of an asynchronous batch: outputs are device pointers. */
@@ -148,8 +148,10 @@ typedef struct {
 typedef struct {
     int32_t tile_w, tile_h;       /* raster tile in pixels: 8, 16, 32 or 64 */
     int64_t refs_capacity;        /* tile-list entries per batch (grown on demand) */
-    int32_t raster_waves;         /* waves per raster workgroup: 1, 2, 4 */
-    int32_t reserved;
+    int32_t raster_waves;         /* waves per raster workgroup: 4 */
+    int32_t tile_split;           /* tile-list entries per raster workgroup, 64..2^24: a
+                                     longer list is split into parts that several
+                                     workgroups resolve (same result, more parallelism) */
 } gr_config;
 
 typedef struct gr_context gr_context;

@@ -79,3 +79,40 @@ def test_config5_resident(ctx):
         check(f"config5-{k}", req, ctx.draw(req, mh), S.subscene(req))
     for k, got in enumerate(draw_batch(ctx, rs.requests, [mh] * len(rs.requests))):
         check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
+
+
+@pytest.mark.parametrize("tile,split", [((32, 32), 64), ((16, 16), 64), ((64, 64), 64)])
+def test_config4_split_tiles(ctx, tile, split):
+    """Tile lists longer than tile_split entries are resolved by several raster
+    workgroups and merged: the frames stay bit-exact. At 64x64 tiles the slab
+    budget runs out and the remaining long lists stay whole."""
+    before = ctx.config()
+    ctx.set_config(tile_w=tile[0], tile_h=tile[1], tile_split=split)
+    try:
+        frames = config4_batch()
+        handles = [ctx.upload_scene(s)[0] for s in frames]
+        res = draw_batch(ctx, frames, handles)
+        split_tiles = sum(r[2]["tiles_split"] for r in res)
+        print(f"tile {tile} split {split}: {split_tiles} tiles split", flush=True)
+        assert split_tiles > 0
+        for k, got in enumerate(res):
+            check(f"config4-{k}", frames[k], got)
+    finally:
+        ctx.set_config(tile_w=before.tile_w, tile_h=before.tile_h,
+                       tile_split=before.tile_split)
+
+
+def test_config5_split_tiles(ctx):
+    """The serving batch with every list longer than 64 entries split."""
+    before = ctx.config()
+    ctx.set_config(tile_split=64)
+    try:
+        rs = config5_set()
+        th = [ctx.upload_texture(t) for t in rs.textures]
+        mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
+        res = draw_batch(ctx, rs.requests, [mh] * len(rs.requests))
+        assert sum(r[2]["tiles_split"] for r in res) > 0
+        for k, got in enumerate(res):
+            check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
+    finally:
+        ctx.set_config(tile_split=before.tile_split)
