@@ -113,11 +113,53 @@ def other_configs(ctx, scenes):
     c4 = scenes.config4_frames(16)
     h4 = [ctx.upload_scene(s)[0] for s in c4]
     run("config4_16_mixed_frames", c4, h4, 40, True)
-    c5 = scenes.config5(8)
+    c5 = scenes.config5(64)
     th = [ctx.upload_texture(t) for t in c5.textures]
     mh = [ctx.upload_mesh(m, th) for m in c5.meshes]
-    run("config5_8_requests_8M_resident", c5.requests, [mh] * len(c5.requests), 20, True)
+    run("config5_8_requests_8M_resident", c5.requests[:8], [mh] * 8, 20, True)
+    out["config5_serving"] = serving(ctx, c5.requests, mh)
     return out
+
+
+def serving(ctx, requests, mh):
+    """Serving form on the config-5 resident set: 64 distinct requests (1-8
+    objects, 640x360 to 1920x1080), latency from submission to colour and depth
+    in host memory. 'serial' draws one request at a time (gr_draw: launch, wait,
+    copy out); 'saturated' submits a burst of 512 at once; the qps lines are
+    open-loop Poisson streams of the stated rate (gr_server_replay)."""
+    import numpy as np
+    from gorender_amd.serve import PreparedRequest, Server, poisson_arrivals, replay
+
+    def summary(lat, wall, n, bs=None):
+        d = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "requests": int(n), "completed_per_s": round(n / wall, 1)}
+        if bs is not None:
+            d["mean_batch"] = round(float(np.mean(bs)), 2)
+        return d
+
+    for s in requests[:8]:  # warm-up: capacity and allocations
+        ctx.draw(s, mh)
+    lat = []
+    t0 = time.perf_counter()
+    for s in requests:
+        t = time.perf_counter()
+        ctx.draw(s, mh)
+        lat.append((time.perf_counter() - t) * 1e3)
+    res = {"serial": summary(lat, time.perf_counter() - t0, len(requests))}
+    reqs = [PreparedRequest(s, mh) for s in requests]
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2) as srv:
+        replay(srv, reqs, np.zeros(64))  # warm-up: target pools
+        t = time.perf_counter()
+        lat, bs = replay(srv, reqs, np.zeros(512))
+        res["saturated"] = summary(lat, time.perf_counter() - t, 512, bs)
+        for qps, n in ((100, 200), (1000, 1000), (10000, 2000)):
+            t = time.perf_counter()
+            lat, bs = replay(srv, reqs, poisson_arrivals(n, qps, seed=qps))
+            res[f"qps_{qps}"] = summary(lat, time.perf_counter() - t, n, bs)
+    res["config"] = {"target_budget_bytes": 2 << 30, "max_batch": 16, "max_inflight": 2,
+                     "distinct_requests": len(requests)}
+    return res
 
 
 def pct(xs, p):

@@ -21,6 +21,7 @@ GR_ERR_INDEX_OUT_OF_RANGE = 5
 GR_ERR_UNSUPPORTED = 6
 GR_ERR_CAPACITY = 7
 GR_ERR_CLIP_OVERFLOW = 8
+GR_ERR_TIMEOUT = 9
 
 GR_SHOW_FACES = 1 << 0
 GR_BACKFACE_CULLING = 1 << 1
@@ -83,6 +84,17 @@ class gr_config(Structure):
                 ("raster_waves", c_int32), ("tile_split", c_int32)]
 
 
+class gr_server_config(Structure):
+    _fields_ = [("target_budget", c_int64), ("max_batch", c_int32), ("max_inflight", c_int32)]
+
+
+class gr_serve_result(Structure):
+    _fields_ = [("status", c_int32), ("width", c_int32), ("height", c_int32),
+                ("batch_size", c_int32), ("rgba", c_void_p), ("depth", c_void_p),
+                ("stats", gr_stats), ("submit_ns", c_int64), ("start_ns", c_int64),
+                ("done_ns", c_int64)]
+
+
 # name -> (restype, argtypes); every symbol of include/gorender_hip.h
 SIGNATURES = {
     "gr_create": (c_int32, [c_int32, POINTER(c_void_p)]),
@@ -139,6 +151,14 @@ SIGNATURES = {
     "gr_kernel_times": (c_int32, [c_void_p, POINTER(c_char_p), POINTER(c_double),
                                   POINTER(c_int64), c_int32]),
     "gr_reset_kernel_times": (None, [c_void_p]),
+    "gr_server_create": (c_int32, [c_void_p, POINTER(gr_server_config), POINTER(c_void_p)]),
+    "gr_server_destroy": (None, [c_void_p]),
+    "gr_server_submit": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32,
+                                   POINTER(c_int64)]),
+    "gr_server_wait": (c_int32, [c_void_p, c_int64, c_int32, POINTER(gr_serve_result)]),
+    "gr_server_release": (c_int32, [c_void_p, c_int64]),
+    "gr_server_replay": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                   c_void_p, c_int32, c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -14,6 +14,7 @@
 
 #include "gorender_hip.h"
 #include "gr_error.h"
+#include "gr_internal.h"
 #include "gr_device.h"
 #include "gr_launch.h"
 
@@ -69,9 +70,7 @@ struct TimedLaunch {
 
 }  // namespace
 
-namespace gr_internal {
-gr_status fail(gr_status s, const std::string& msg) { return ::fail(s, msg); }
-}  // namespace gr_internal
+
 
 struct gr_context {
     int device = 0;
@@ -596,6 +595,43 @@ bool is_pow2(int32_t x) { return x != 0 && (x & (x - 1)) == 0; }
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+
+namespace gr_internal {
+
+gr_status fail(gr_status s, const std::string& msg) { return ::fail(s, msg); }
+
+gr_status enqueue_batch(gr_context* c, const gr_batch_frame* frames, int32_t nf,
+                        const gr_object* objs, int32_t no) {
+    try {
+        GR_HIP(hipSetDevice(c->device));
+        return enqueue(c, frames, nf, objs, no);
+    } catch (...) {
+        return ::fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+hipStream_t context_stream(gr_context* c) { return c->stream; }
+int context_device(gr_context* c) { return c->device; }
+const void* last_stats_device(gr_context* c) { return c->last_stats; }
+size_t stats_bytes_per_frame() { return gr::kNumStats * sizeof(unsigned long long); }
+void decode_stats(const void* frame_stats, gr_stats* out) {
+    to_gr_stats((const unsigned long long*)frame_stats, out);
+}
+void* control_device(gr_context* c) { return c->ctrl.p; }
+size_t control_bytes() { return sizeof(gr::DevControl); }
+void decode_control(const void* host_copy, bool* overflow, unsigned long long* refs_needed,
+                    bool* unsupported) {
+    gr::DevControl ctl;
+    std::memcpy(&ctl, host_copy, sizeof ctl);
+    *overflow = ctl.overflow != 0;
+    *refs_needed = ctl.refs_needed;
+    *unsupported = ctl.unsupported != 0;
+}
+void grow_refs_to(gr_context* c, unsigned long long needed) { grow_refs(c, needed); }
+bool mesh_live(gr_context* c, int32_t h) {
+    return h >= 0 && h < (int32_t)c->meshes.size() && c->meshes[h].live;
+}
+
+}  // namespace gr_internal
 
 extern "C" {
 

@@ -1,0 +1,31 @@
+// Library-internal access to a context for the serving loop (gr_serve.cpp).
+// Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "gorender_hip.h"
+
+namespace gr_internal {
+
+// gr_draw_batch without the argument checks: queue one batch on the context
+// stream. The stats of its frames then live at last_stats_device() until the
+// next batch is queued.
+gr_status enqueue_batch(gr_context* c, const gr_batch_frame* frames, int32_t nf,
+                        const gr_object* objs, int32_t no);
+hipStream_t context_stream(gr_context* c);
+int context_device(gr_context* c);
+const void* last_stats_device(gr_context* c);
+size_t stats_bytes_per_frame();
+void decode_stats(const void* frame_stats, gr_stats* out);
+// Sticky control words (device) and their size; decode a host copy of them.
+void* control_device(gr_context* c);
+size_t control_bytes();
+void decode_control(const void* host_copy, bool* overflow, unsigned long long* refs_needed,
+                    bool* unsupported);
+// Tile-list capacity for the next batch, after an overflow that needed `needed`.
+void grow_refs_to(gr_context* c, unsigned long long needed);
+bool mesh_live(gr_context* c, int32_t handle);
+
+}  // namespace gr_internal

@@ -16,6 +16,11 @@
  *                               FrameBuffer.Pixels / ZBuffer (rasterizer.go:7-23) and TPF
  *   gr_draw_device / gr_draw_batch   the same Draw, asynchronous, device-resident
  *                               outputs (no reference counterpart: batching form)
+ *   gr_server_*                 a continuous-batching request loop over resident
+ *                               meshes and textures, frames read back to pinned
+ *                               host memory (no reference counterpart: serving form;
+ *                               replaces the frame loop of main.go:222-227 for
+ *                               many independent requests)
  *
  * Output semantics: the image is the reference's sequential draw order
  * (renderer.go:466-474): objects, then faces, then clipped fan triangles.
@@ -32,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GR_ABI_VERSION 1
+#define GR_ABI_VERSION 2
 
 typedef enum {
     GR_OK = 0,
@@ -50,6 +55,8 @@ typedef enum {
     /* Clip polygon exceeded 9 vertices (reference: index out of range panic,
      * clipping.go:35-40); the face is dropped and counted in gr_stats. */
     GR_ERR_CLIP_OVERFLOW = 8,
+    /* gr_server_wait: the request did not finish within the timeout. */
+    GR_ERR_TIMEOUT = 9,
 } gr_status;
 
 /* Renderer toggles (renderer.go:90-97). Defaults of NewRenderer
@@ -257,6 +264,55 @@ gr_status gr_png_decode(const char *path, int32_t *width, int32_t *height, uint8
 void gr_host_free(void *ptr);
 /* LoadTextureFile + gr_texture_create (type from gr_texture_kind_for_size). */
 gr_status gr_texture_load(gr_context *ctx, const char *path, int32_t *out_handle);
+
+/* ---- serving (no reference counterpart) ----
+ * A request loop on one context: requests are admitted in arrival order while
+ * their colour + depth targets (width * height * 8 bytes each) fit the budget,
+ * every admitted request joins the next batched launch, and each frame is read
+ * back into pinned host memory on a second stream while later batches draw.
+ * A batch that overflows its tile lists is drawn again with more capacity.
+ * Frames are bit-identical to gr_draw of the same request alone. One worker
+ * thread drives the context: upload meshes and textures first, and do not use
+ * the context directly while the server runs. */
+typedef struct gr_server gr_server;
+typedef struct {
+    int64_t target_budget; /* bytes of device colour + depth targets admitted at once */
+    int32_t max_batch;     /* requests per batched launch */
+    int32_t max_inflight;  /* launches queued on the device at once */
+} gr_server_config;
+typedef struct {
+    int32_t status;        /* gr_status of the request */
+    int32_t width, height;
+    int32_t batch_size;    /* requests in the launch that drew it */
+    const uint8_t *rgba;   /* [H][W][4], pinned host memory valid until gr_server_release */
+    const float *depth;    /* [H][W] */
+    gr_stats stats;
+    int64_t submit_ns;     /* steady clock: submitted */
+    int64_t start_ns;      /* its batch was launched */
+    int64_t done_ns;       /* colour and depth are in host memory */
+} gr_serve_result;
+gr_status gr_server_create(gr_context *ctx, const gr_server_config *cfg, gr_server **out);
+/* Finishes every submitted request, then frees the server (results included). */
+void gr_server_destroy(gr_server *server);
+/* Queue a request (frame and objects are copied); a frame larger than the
+ * budget -> GR_ERR_INVALID_ARGUMENT, a dead mesh handle -> GR_ERR_INVALID_HANDLE. */
+gr_status gr_server_submit(gr_server *server, const gr_frame *frame, const gr_object *objects,
+                           int32_t num_objects, int64_t *ticket);
+/* Wait for a request (timeout_ms < 0: no limit; GR_ERR_TIMEOUT when it passes)
+ * and describe its result; returns the request's status. */
+gr_status gr_server_wait(gr_server *server, int64_t ticket, int32_t timeout_ms,
+                         gr_serve_result *out);
+/* Hand a finished request's host buffers back to the server. */
+gr_status gr_server_release(gr_server *server, int64_t ticket);
+/* Open-loop load for measurement: submits request i % num_distinct (frame d
+ * with objects[first_object[d] .. + num_objects[d]]) at arrival_s[i] seconds
+ * after the call starts (busy-waiting on the clock), while a second thread
+ * waits for and releases results in submission order; latency_ns[i]
+ * (submit -> colour and depth in host memory) and batch_size[i] may be NULL. */
+gr_status gr_server_replay(gr_server *server, const gr_frame *frames, const gr_object *objects,
+                           const int32_t *first_object, const int32_t *num_objects,
+                           int32_t num_distinct, const double *arrival_s, int32_t n,
+                           int64_t *latency_ns, int32_t *batch_size);
 
 /* ---- measurement ---- */
 /* When enabled, each kernel launch is bracketed by HIP events on the context

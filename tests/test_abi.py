@@ -43,7 +43,7 @@ def test_struct_layout_matches_c(tmp_path):
     """Compile the header as C99 and compare sizeof/offsetof with ctypes."""
     src = tmp_path / "sizes.c"
     structs = ["gr_texture_desc", "gr_mesh_desc", "gr_object", "gr_frame", "gr_stats",
-               "gr_batch_frame", "gr_config"]
+               "gr_batch_frame", "gr_config", "gr_server_config", "gr_serve_result"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gorender_hip.h"',
              "int main(void) {"]
     for s in structs:
@@ -66,7 +66,7 @@ def test_struct_layout_matches_c(tmp_path):
 
 def test_abi_version_and_defaults():
     lib = _lib.load()
-    assert lib.gr_abi_version() == 1
+    assert lib.gr_abi_version() == 2
     f = _lib.gr_frame()
     lib.gr_frame_defaults(ctypes.byref(f), 800, 600)
     # NewRenderer / Draw defaults (renderer.go:12-17,118-136,448-449)

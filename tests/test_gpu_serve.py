@@ -1,0 +1,99 @@
+"""Serving loop (gr_server_*): every request's frame and counters equal the
+oracle's for that request drawn alone (committed digests), whatever batch it
+shares, under tight target budgets, and after a tile-list overflow makes the
+server grow its capacity and draw a batch again."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gorender_amd import _lib
+from gorender_amd import scenes as S
+from gorender_amd.serve import PreparedRequest, Server, poisson_arrivals, replay
+from parity import digest
+from scene_cases import config5_set
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracle_digests.json")))
+
+
+@pytest.fixture(scope="module")
+def resident(ctx):
+    rs = config5_set()
+    th = [ctx.upload_texture(t) for t in rs.textures]
+    mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
+    return rs, [PreparedRequest(r, mh) for r in rs.requests]
+
+
+def check_result(k, res):
+    assert res.rgba is not None
+    got = digest(res.rgba, res.depth, res.stats)
+    assert got == GOLDEN[f"config5-{k}"], f"config5-{k}: served frame differs from the oracle"
+    assert res.done_ns >= res.start_ns >= res.submit_ns
+
+
+@pytest.mark.parametrize("budget_mib,max_batch,max_inflight", [(1024, 8, 2), (40, 3, 2),
+                                                                (17, 1, 1)])
+def test_mix_equals_alone(ctx, resident, budget_mib, max_batch, max_inflight):
+    """Each of the 8 requests is submitted twice in a shuffled order; all are
+    in flight together and batched as the budget allows (17 MiB holds a single
+    1920x1080 request)."""
+    rs, reqs = resident
+    order = list(range(len(reqs))) * 2
+    np.random.default_rng(budget_mib).shuffle(order)
+    print(f"budget {budget_mib} MiB, max_batch {max_batch}, max_inflight {max_inflight}",
+          flush=True)
+    with Server(ctx, budget_mib << 20, max_batch, max_inflight) as srv:
+        tickets = [srv.submit(reqs[k]) for k in order]
+        results = [srv.wait(t) for t in tickets]
+    sizes = [r.batch_size for r in results]
+    assert max(sizes) <= max_batch
+    if max_batch > 1:
+        assert max(sizes) > 1, "requests in flight together were never batched"
+    for k, res in zip(order, results):
+        check_result(k, res)
+
+
+def test_overflow_regrows_and_redraws(ctx, resident):
+    """A tile-list capacity far below what the requests need: the server grows
+    it and draws the overflowing batches again."""
+    rs, reqs = resident
+    ctx.set_config(refs_capacity=1 << 16)
+    with Server(ctx, 1 << 30, 8, 2) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    assert ctx.config().refs_capacity > 1 << 16
+    for k, res in enumerate(results):
+        check_result(k, res)
+
+
+def test_rejects_bad_requests(ctx, resident):
+    rs, reqs = resident
+    budget = 4 << 20  # holds 640x360 (1.8 MB), not 1280x720 (7.4 MB)
+    with Server(ctx, budget, 4, 2) as srv:
+        big = next(r for r in reqs if r.scene.frame.width * r.scene.frame.height * 8 > budget)
+        with pytest.raises(_lib.GorenderError) as e:
+            srv.submit(big)
+        assert e.value.status == _lib.GR_ERR_INVALID_ARGUMENT
+        small = next(s for s in rs.requests if s.frame.width * s.frame.height * 8 <= budget)
+        bad = PreparedRequest(small, [10_000] * len(rs.meshes))
+        with pytest.raises(_lib.GorenderError) as e:
+            srv.submit(bad)
+        assert e.value.status == _lib.GR_ERR_INVALID_HANDLE
+        with pytest.raises(_lib.GorenderError) as e:
+            srv.wait_raw(12345)
+        assert e.value.status == _lib.GR_ERR_INVALID_HANDLE
+
+
+def test_open_loop_replay(ctx, resident):
+    """Native open-loop driver: every request completes, latencies are
+    positive and some requests share a launch at 2000 requests/s."""
+    rs, reqs = resident
+    with Server(ctx, 1 << 30, 16, 2) as srv:
+        lat, bs = replay(srv, reqs, poisson_arrivals(96, 2000.0, seed=7))
+    assert len(lat) == 96 and (lat > 0).all()
+    assert bs.min() >= 1 and bs.max() > 1
+    print(f"replay p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms "
+          f"mean batch {bs.mean():.2f}", flush=True)
