@@ -1372,12 +1372,13 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// int32 in the constant address space: uniform loads through it become
-// scalar loads. Only for memory no thread writes during the kernel.
-typedef const __attribute__((address_space(4))) int32_t kint;
+// The raster stage reads a staged RasterRec as five 16-byte pieces: divisors,
+// reciprocals, then (box x, box y, wide, A01), (B01, C01, A12, B12),
+// (C12, A20, B20, C20).
 static_assert(offsetof(RasterRec, bx0) == 32 && offsetof(RasterRec, by0) == 36 &&
-                  offsetof(RasterRec, wide) == 40 && offsetof(RasterRec, e) == 44,
-              "scalar loads index RasterRec as int32 words");
+                  offsetof(RasterRec, wide) == 40 && offsetof(RasterRec, e) == 44 &&
+                  sizeof(RasterRec) == 80,
+              "raster records are staged as five uint4 pieces");
 
 // Load of a 16-byte-aligned record through a global-address-space pointer
 // (global_load rather than flat, so LDS traffic is not waited on with it).
@@ -1399,7 +1400,7 @@ __device__ __forceinline__ T gload(const T* p) {
 // one per lane, and the (depth, sequence) max into the tile key.
 template <int TW>
 __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t head,
-                                           uint32_t count, const float4 (*ldiv)[2],
+                                           uint32_t count, const uint4* lrec,
                                            const uint32_t* lslot, unsigned long long* key,
                                            int lane) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1410,7 +1411,12 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
         const uint32_t pk = q[0][k];
         const int32_t f12 = (int32_t)q[1][k], f20 = (int32_t)q[2][k];
         const uint32_t i = pk >> 12, px = (pk >> 6) & 63u, py = pk & 63u;
-        const Bary b = bary((float)f12, (float)f20, ldiv[i][0], ldiv[i][1]);
+        const uint4 d = lrec[i * 5], rc = lrec[i * 5 + 1];
+        const Bary b = bary((float)f12, (float)f20,
+                            make_float4(__uint_as_float(d.x), __uint_as_float(d.y),
+                                        __uint_as_float(d.z), __uint_as_float(d.w)),
+                            make_float4(__uint_as_float(rc.x), __uint_as_float(rc.y),
+                                        __uint_as_float(rc.z), __uint_as_float(rc.w)));
         if (b.zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(b.zrec, lslot[i] + 1u));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1418,10 +1424,10 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS of k_raster beyond the key array: 64 staged records and slots, the
-// per-wave fragment rings and the stats reduction.
+// LDS of k_raster beyond the key array: 64 staged records (80 B) and slots,
+// the per-wave fragment rings and the stats reduction.
 constexpr int kRasterChunk = 64;
-constexpr int kRasterLdsFixed = kRasterChunk * (32 + 4) + 4 * 3 * 128 * 4 + 4 * 2 * 8;
+constexpr int kRasterLdsFixed = kRasterChunk * (80 + 4) + 4 * 3 * 128 * 4 + 4 * 2 * 8;
 
 // Waves per SIMD the tile's LDS allows (one wave per SIMD per workgroup), at
 // most 8: the register budget follows from it (512 / waves VGPRs).
@@ -1437,22 +1443,22 @@ constexpr int raster_waves(int npx) {
 template <int TW, int TH>
 __device__ __forceinline__ unsigned long long raster_list(
     const DevFrame& fr, int X0, int Y0, const uint32_t* __restrict__ list, uint32_t cnt,
-    unsigned long long* key, float4 (*ldiv)[2], uint32_t* lslot, uint32_t (*q)[128], int lane,
+    unsigned long long* key, uint4* lrec, uint32_t* lslot, uint32_t (*q)[128], int lane,
     int wave) {
     constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
     unsigned long long covered_lane = 0;  // wide path, per lane
     uint32_t covered_wave = 0;            // int32 path, wave-uniform
     for (uint32_t base = 0; base < cnt; base += CHUNK) {
         const uint32_t n = min(cnt - base, (uint32_t)CHUNK);
-        // Stage the chunk: slots (one coalesced read), then the divisors and
-        // reciprocals of the raster records as 16-byte pieces.
+        // Stage the chunk: slots (one coalesced read), then the raster records
+        // as 16-byte pieces, all loads of the chunk in flight together.
         __syncthreads();
         if (threadIdx.x < n) lslot[threadIdx.x] = list[base + threadIdx.x];
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < 2 * n; p += 256) {
-            const uint32_t ri = p >> 1, pi = p & 1u;
+        for (uint32_t p = threadIdx.x; p < 5 * n; p += 256) {
+            const uint32_t ri = p / 5u, pi = p - ri * 5u;
             const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[ri]]);
-            reinterpret_cast<uint4*>(&ldiv[ri][0])[pi] = gload(src + pi);
+            lrec[p] = gload(src + pi);
         }
         __syncthreads();
         // Per wave, one triangle at a time: lanes cover an lpr x (64/lpr) pixel
@@ -1461,13 +1467,11 @@ __device__ __forceinline__ unsigned long long raster_list(
         // to an LDS ring by ballot and their exact depth runs on full waves.
         uint32_t qhead = 0, qtail = 0;  // wave-uniform ring positions
         for (uint32_t i = wave; i < n; i += 4) {
-            // The record's box and edge coefficients are wave-uniform: scalar
-            // loads through the constant address space (the records are
-            // read-only here).
-            const uint32_t rslot = (uint32_t)sgpr((int)lslot[i]);
-            const kint* rk = (const kint*)(const void*)&fr.rrec[rslot];
-            const uint32_t bxw = (uint32_t)rk[8];
-            const uint32_t byw = (uint32_t)rk[9];
+            // The record's box and edge coefficients are wave-uniform: read
+            // from LDS and moved to scalar registers.
+            const uint4 w2 = lrec[i * 5 + 2], w3 = lrec[i * 5 + 3], w4 = lrec[i * 5 + 4];
+            const uint32_t bxw = (uint32_t)sgpr((int)w2.x);
+            const uint32_t byw = (uint32_t)sgpr((int)w2.y);
             const int x0 = max((int)(int16_t)(bxw & 0xFFFFu), X0);
             const int x1 = min((int)(int16_t)(bxw >> 16), X0 + TW - 1);
             const int y0 = max((int)(int16_t)(byw & 0xFFFFu), Y0);
@@ -1477,11 +1481,16 @@ __device__ __forceinline__ unsigned long long raster_list(
             if (l2 > 6) l2 = 6;
             const int lpr = 1 << l2, rows = 64 >> l2;
             const int lx = lane & (lpr - 1), ly = lane >> l2;
-            if (rk[10]) {
+            if (sgpr((int)w2.z)) {
                 // Coordinates beyond the int32 range: direct int64 evaluation.
-                const uint32_t slot = rslot;
+                const uint32_t slot = (uint32_t)sgpr((int)lslot[i]);
                 const WideRec* wr = &fr.wrec[slot];
-                const float4* rp = ldiv[i];
+                const uint4 d4 = lrec[i * 5], r4 = lrec[i * 5 + 1];
+                const float4 rp[2] = {
+                    make_float4(__uint_as_float(d4.x), __uint_as_float(d4.y),
+                                __uint_as_float(d4.z), __uint_as_float(d4.w)),
+                    make_float4(__uint_as_float(r4.x), __uint_as_float(r4.y),
+                                __uint_as_float(r4.z), __uint_as_float(r4.w))};
                 for (int yb = y0; yb <= y1; yb += rows) {
                     const int y = yb + ly;
                     for (int xb = x0; xb <= x1; xb += lpr) {
@@ -1499,9 +1508,9 @@ __device__ __forceinline__ unsigned long long raster_list(
                 }
                 continue;
             }
-            const int A0 = rk[11], B0 = rk[12], C0 = rk[13];
-            const int A1 = rk[14], B1 = rk[15], C1 = rk[16];
-            const int A2 = rk[17], B2 = rk[18], C2 = rk[19];
+            const int A0 = sgpr((int)w2.w), B0 = sgpr((int)w3.x), C0 = sgpr((int)w3.y);
+            const int A1 = sgpr((int)w3.z), B1 = sgpr((int)w3.w), C1 = sgpr((int)w4.x);
+            const int A2 = sgpr((int)w4.y), B2 = sgpr((int)w4.z), C2 = sgpr((int)w4.w);
             // |A|, |B| <= 2^15 and |x|, |y| < 2^14 on this path: 24-bit products
             // are exact and every edge value fits int32.
             const int px = x0 + lx, py = y0 + ly;
@@ -1537,7 +1546,7 @@ __device__ __forceinline__ unsigned long long raster_list(
                         qtail += m;
                         covered_wave += m;
                         if (qtail - qhead >= 64u) {
-                            frag_depth<TW>(q, qhead, 64u, ldiv, lslot, key, lane);
+                            frag_depth<TW>(q, qhead, 64u, lrec, lslot, key, lane);
                             qhead += 64u;
                         }
                     }
@@ -1554,7 +1563,7 @@ __device__ __forceinline__ unsigned long long raster_list(
         }
         while (qtail != qhead) {
             const uint32_t cnt_q = min(qtail - qhead, 64u);
-            frag_depth<TW>(q, qhead, cnt_q, ldiv, lslot, key, lane);
+            frag_depth<TW>(q, qhead, cnt_q, lrec, lslot, key, lane);
             qhead += cnt_q;
         }
     }
@@ -1669,7 +1678,7 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
                     uint32_t part_refs) {
     constexpr int NPX = TW * TH;
     __shared__ unsigned long long key[NPX];
-    __shared__ float4 ldiv[kRasterChunk][2];
+    __shared__ uint4 lrec[kRasterChunk * 5];
     __shared__ uint32_t lslot[kRasterChunk];
     __shared__ uint32_t qbuf[4][3][128];
     __shared__ unsigned long long red[4];
@@ -1712,7 +1721,7 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
         // litem) before what follows.
         unsigned long long covered =
             raster_list<TW, TH>(fr, X0, Y0, refs + tile_offsets[tile] + beg,
-                                min(cnt - beg, part_refs), key, ldiv, lslot, qbuf[wave], lane,
+                                min(cnt - beg, part_refs), key, lrec, lslot, qbuf[wave], lane,
                                 wave);
         __syncthreads();
         unsigned long long* slab = slabs + (size_t)tile_slab[tile] * NPX;
@@ -1743,7 +1752,7 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
               unsigned long long* __restrict__ slabs, uint32_t part_refs) {
     constexpr int NPX = TW * TH;
     __shared__ unsigned long long key[NPX];
-    __shared__ float4 ldiv[kRasterChunk][2];  // divisors and reciprocals of the staged records
+    __shared__ uint4 lrec[kRasterChunk * 5];  // the staged raster records
     __shared__ uint32_t lslot[kRasterChunk];
     __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
     __shared__ unsigned long long red[4][2];
@@ -1760,7 +1769,7 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
     const uint32_t slab_i = cnt > part_refs ? tile_slab[g] : kNoSlab;
     // raster_list's first barrier orders the initialisation before its use.
     const unsigned long long covered = raster_list<TW, TH>(
-        fr, X0, Y0, refs + off, slab_i != kNoSlab ? part_refs : cnt, key, ldiv, lslot, qbuf[wave],
+        fr, X0, Y0, refs + off, slab_i != kNoSlab ? part_refs : cnt, key, lrec, lslot, qbuf[wave],
         lane, wave);
     __syncthreads();
     if (slab_i != kNoSlab) {
