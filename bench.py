@@ -38,8 +38,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: no CPU baseline, no latency loops")
-    ap.add_argument("--mode", choices=["both", "batched", "plain"], default="both",
-                    help="forms to run (rocprofv3 runs profile one form at a time)")
+    ap.add_argument("--mode", choices=["both", "batched", "plain", "config5"], default="both",
+                    help="forms to run (rocprofv3 runs profile one form at a time); config5 "
+                         "only launches the 8-request serving batch --steps times")
     ap.add_argument("--tile", default="32x32",
                     help="raster tile WxH, or 'auto' to time the candidates first and keep "
                          "the fastest")
@@ -162,6 +163,34 @@ def serving(ctx, requests, mh):
     return res
 
 
+def profile_config5(steps, warmup, tile_split):
+    """The config-5 batch (8 requests over the 32 resident meshes) launched
+    `steps` times, for kernel traces of the split-tile path."""
+    from gorender_amd import scenes
+    from gorender_amd.device import Context
+    ctx = Context(0)
+    ctx.set_config(tile_split=tile_split)
+    rs = scenes.config5(8)
+    th = [ctx.upload_texture(t) for t in rs.textures]
+    mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
+    bufs = [ctx.frame_buffers(s.frame.width, s.frame.height) for s in rs.requests]
+    pb = ctx.prepare_batch(rs.requests, [mh] * 8, [(b[0].ptr, b[1].ptr) for b in bufs])
+    for _ in range(max(2, warmup)):
+        ctx.submit_batch(pb)
+        try:
+            ctx.synchronize()
+        except Exception:
+            pass  # the first launch grows the tile-list capacity
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.submit_batch(pb)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    st = ctx.last_stats()
+    print(json.dumps({"mode": "config5", "launches": steps, "ms_per_launch": round(dt / steps * 1e3, 4),
+                      "tiles_split": sum(x["tiles_split"] for x in st)}))
+
+
 def pct(xs, p):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(math.ceil(p / 100.0 * len(xs))) - 1))
@@ -178,6 +207,10 @@ def main():
     import numpy as np
     from gorender_amd import scenes
     from gorender_amd.device import Context
+
+    if args.mode == "config5":
+        profile_config5(args.steps, args.warmup, args.tile_split)
+        return
 
     scene = scenes.config2()
     fr = scene.frame

@@ -794,6 +794,27 @@ __device__ __forceinline__ void finalize_record(RasterRec* r, ShadeRec* sr) {
 }
 
 // Exclusive block scan of small counts (256 threads = 4 waves).
+// Exclusive scan over the workgroup of per-face fan sizes, which are 0, 1, 2
+// or kMaxFan: the in-wave prefix is counted from three ballots (no cross-lane
+// shuffles, whose lane addresses would need registers at the point where
+// k_face holds the most), the wave totals go through LDS.
+__device__ __forceinline__ int fan_exclusive_scan(int v, int* lds4, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t b1 = __ballot(v == 1), b2 = __ballot(v == 2), b7 = __ballot(v == kMaxFan);
+    const uint64_t lt = lanemask_lt(lane);
+    const int x = __popcll(b1 & lt) + 2 * __popcll(b2 & lt) + kMaxFan * __popcll(b7 & lt);
+    if (lane == 0) lds4[wave] = __popcll(b1) + 2 * __popcll(b2) + kMaxFan * __popcll(b7);
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int w = 0; w < 4; w++) {
+        const int t = lds4[w];
+        if (w < wave) base += t;
+        total += t;
+    }
+    return base + x;
+}
+
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total) {
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
@@ -933,14 +954,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
     // Pass 2, survivor s on thread s: lighting, texture coordinates, clipping.
     // 0: nothing, 1: emit the triangle as is, 2: full clipper in the wave,
-    // 3: one-plane polygon in q[0..nq-1]
+    // 3: one-plane polygon. q[0..nq-1] holds the polygon to emit (the
+    // triangle itself for modes 1 and 2), so only one set of corners stays
+    // live across the emits.
     int mode = 0;
     int n = 0;
-    CV c0, c1, c2;
     CV q[4];
     int nq = 0;
     int tex = -1;
     if ((int)threadIdx.x < nsurv) {
+        CV c0, c1, c2;
         const uint32_t e = lds_sface[threadIdx.x];
         const int sf = wg_in_obj * kFacesPerWg + (int)(e & 0xFFu);
         const int sp = (int)(e >> 8);
@@ -995,6 +1018,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             // unchanged (clipping.go:199-232); no clipping: as is.
             mode = 1;
             n = 1;
+            nq = 3;
+            q[0] = c0;
+            q[1] = c1;
+            q[2] = c2;
         } else {
             nq = clip_one_plane(sp, c0, c1, c2, fr.znear, fr.zfar, q);
             if (nq > 0) {
@@ -1003,6 +1030,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             } else if (nq < 0) {
                 nq = 0;
                 mode = 2;  // counted below, one face at a time per wave
+                q[0] = c0;
+                q[1] = c1;
+                q[2] = c2;
             }
         }
     }
@@ -1012,16 +1042,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     // binned, so slots stay in draw order and each face is clipped once.
     if (mode == 2) n = kMaxFan;
     int total;
-    const int off = block_exclusive_scan(n, lds_scan, total);
+    const int off = fan_exclusive_scan(n, lds_scan, total);
     const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
     const uint32_t slot0 = wg_slot0 + (uint32_t)off;
-    if (mode == 1) add_emit(st, emit_triangle(fr, c0, c1, c2, tex, slot0, tally, twl, thl));
-    if (mode == 3) {
+    // Fan of q (Polygon.Triangulate, clipping.go:44-62): (P0, P1, P2), then
+    // (P0, P2, P3) for a quad.
+    if (mode == 1 || mode == 3)
         add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tally, twl, thl));
-        if (nq == 4)
-            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl,
-                                       thl));
-    }
+    if (mode == 3 && nq == 4)
+        add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl, thl));
 
     // General faces, kStage at a time per wave: the wave clips each one into
     // its stage (clipping.go:167-236), then one lane per fan slot emits the
@@ -1033,7 +1062,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         int nst = 0;
         for (; gen && nst < kStage; gen &= gen - 1, nst++) {
             const int L = __builtin_ctzll(gen);
-            const CV a = shfl_cv(c0, L), bb = shfl_cv(c1, L), c = shfl_cv(c2, L);
+            const CV a = shfl_cv(q[0], L), bb = shfl_cv(q[1], L), c = shfl_cv(q[2], L);
             const int texL = __shfl(tex, L, 64);
             const int sL = __shfl((int)slot0, L, 64);
             const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
