@@ -387,8 +387,12 @@ __device__ int clip_one_plane(int p, const CV& c0, const CV& c1, const CV& c2, f
 // Triangle set-up
 // ---------------------------------------------------------------------------
 
+// Per-lane counters of k_face packed in one word (one register for the whole
+// kernel): TPF references in bits 0-15, binned triangles 16-23, front faces
+// 24-27, clip overflows 28-31. A lane owns one face in pass 1 and emits at most
+// a few triangles of at most 16 references each, far below these widths.
 struct FaceStats {
-    unsigned long long tpf, tris, front, overflow;
+    uint32_t packed;
 };
 
 // ---------------------------------------------------------------------------
@@ -709,8 +713,7 @@ __device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t slot) 
 constexpr int kStage = 64 / kMaxFan;
 
 __device__ __forceinline__ void add_emit(FaceStats& st, uint32_t e) {
-    st.tpf += e & 0xFFFFu;
-    st.tris += e >> 16;
+    st.packed += (e & 0xFFFFu) + ((e >> 16) << 16);
 }
 
 // ---------------------------------------------------------------------------
@@ -904,7 +907,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int vis = box_visibility(o, fr, lane);
 
-    FaceStats st = {0, 0, 0, 0};
+    FaceStats st = {0u};
 
     // Pass 1, one face per thread: clip-space corners, backface test and plane
     // classification. The survivors (front faces the clipper does not reject
@@ -919,7 +922,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         v1 = corner(o, i1);
         v2 = corner(o, i2);
         if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
-            st.front++;
+            st.packed += 1u << 24;
             uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
             if ((flags & kFrustumClipping) && vis != kBoxInside) {
                 m0 = inside_mask(CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
@@ -1066,7 +1069,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             const int texL = __shfl(tex, L, 64);
             const int sL = __shfl((int)slot0, L, 64);
             const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
-            if (k < 0 && lane == L) st.overflow++;
+            if (k < 0 && lane == L) st.packed += 1u << 28;
             if (lane < k) lds_stage[wave][nst][lane] = buf0[lane];
             if (lane == 0) {
                 lds_stage_k[wave][nst] = k;
@@ -1096,8 +1099,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
     // Block-reduce the counters, one atomic per counter per block.
     {
-        unsigned long long s0 = wave_sum_u64(st.tpf), s1 = wave_sum_u64(st.tris);
-        unsigned long long s2 = wave_sum_u64(st.front), s3 = wave_sum_u64(st.overflow);
+        unsigned long long s0 = wave_sum_u64(st.packed & 0xFFFFu);
+        unsigned long long s1 = wave_sum_u64((st.packed >> 16) & 0xFFu);
+        unsigned long long s2 = wave_sum_u64((st.packed >> 24) & 0xFu);
+        unsigned long long s3 = wave_sum_u64(st.packed >> 28);
         if (lane == 0) {
             lds_st[wave][0] = s0;
             lds_st[wave][1] = s1;
