@@ -721,6 +721,10 @@ __device__ __forceinline__ void add_emit(FaceStats& st, uint32_t e) {
 // ---------------------------------------------------------------------------
 
 constexpr int kLdsTiles = 1024;  // LDS tile counters per workgroup
+// One wave per k_bin_fill workgroup: a face block emits a few dozen triangles
+// in typical frames, and the workgroup's time is its chain of dependent memory
+// round trips, so more resident workgroups per CU is what makes it faster.
+constexpr int kFillThreads = 64;
 
 struct TileRect {
     int tx0, tx1, ty0, ty1;
@@ -1241,7 +1245,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
 // k_bin_fill
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ objs, int nobj,
+__global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __restrict__ objs, int nobj,
                                                   const DevFrame* __restrict__ frames,
                                                   const uint32_t* __restrict__ wg_count,
                                                   const uint32_t* __restrict__ frame_overflow,
@@ -1258,7 +1262,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
     const uint32_t cnt = wg_count[b];
     if (cnt == 0) return;
     const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256)
+    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads)
         finalize_record(&fr.rrec[slot0 + i], &fr.srec[slot0 + i]);
     // The barrier in block_tile_rect orders these writes before the reads below.
     block_tile_rect(fr.rrec, slot0, cnt, twl, thl, lrect);
@@ -1267,7 +1271,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
     const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
     if (rw * rh > kLdsTiles) {
         // Very large rectangle: one global atomic per reference.
-        for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+        for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
             TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
             for (int ty = t.ty0; ty <= t.ty1; ty++)
                 for (int tx = t.tx0; tx <= t.tx1; tx++) {
@@ -1278,9 +1282,9 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
         }
         return;
     }
-    for (int i = threadIdx.x; i < rw * rh; i += 256) lcnt[i] = 0u;
+    for (int i = threadIdx.x; i < rw * rh; i += kFillThreads) lcnt[i] = 0u;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
         TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++)
@@ -1288,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
     }
     __syncthreads();
     // One global reservation per touched tile; lcnt becomes the write cursor.
-    for (int i = threadIdx.x; i < rw * rh; i += 256) {
+    for (int i = threadIdx.x; i < rw * rh; i += kFillThreads) {
         const uint32_t v = lcnt[i];
         if (v)
             lcnt[i] = atomicAdd(&tile_cursor[fr.tile_begin + (ry0 + i / rw) * fr.tiles_x + rx0 +
@@ -1296,7 +1300,7 @@ __global__ __launch_bounds__(256) void k_bin_fill(const DevObject* __restrict__ 
                                 v);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
+    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
         TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++) {
@@ -1902,7 +1906,7 @@ void launch_bin_scan(const LaunchParams& p, hipStream_t s) {
 
 void launch_bin_fill(const LaunchParams& p, hipStream_t s) {
     if (p.face_blocks == 0) return;
-    hipLaunchKernelGGL(k_bin_fill, dim3(p.face_blocks), dim3(256), 0, s, p.objs, p.nobj,
+    hipLaunchKernelGGL(k_bin_fill, dim3(p.face_blocks), dim3(kFillThreads), 0, s, p.objs, p.nobj,
                        p.frames, p.wg_count, p.frame_overflow, p.tile_cursor, p.refs,
                        p.tw_log2, p.th_log2);
 }
