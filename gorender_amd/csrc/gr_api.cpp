@@ -346,6 +346,8 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             okey_words += (size_t)fr.width * fr.height;
         }
         int seq = 0;
+        d.obj_begin = (int32_t)dobj.size();
+        d.obj_count = bf[f].num_objects;
         for (int k = 0; k < bf[f].num_objects; k++) {
             const gr_object& ob = objs[bf[f].first_object + k];
             if (ob.mesh < 0 || ob.mesh >= (int)c->meshes.size() || !c->meshes[ob.mesh].live)

@@ -88,7 +88,8 @@ struct DevFrame {
     int32_t tile_begin;          // first tile in the batch tile arrays
     int32_t face_block_begin;    // first k_face block of this frame
     int32_t face_blocks;         // k_face blocks of this frame
-    int32_t pad0;
+    int32_t obj_begin;           // first DevObject of this frame (objects are by frame)
+    int32_t obj_count;
     uint8_t* rgba;               // device outputs
     float* depth;
     uint32_t* okey;              // overlay keys [H*W] or null: (slot+1) << 1 | vertex

@@ -898,8 +898,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         lds_tile_tag[i] = kNoTile;
         lds_tile_cnt[i] = 0u;
     }
+    // Search only this frame's objects (one step for one object per frame).
     const int b = frames[bf].face_block_begin + blocal;
-    const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
+    const int ob0 = frames[bf].obj_begin;
+    const int oi =
+        ob0 + last_le(frames[bf].obj_count, b, [&](int i) { return objs[ob0 + i].face_block; });
     const DevObject& o = objs[oi];
     const TileTally tally{(lds_u32*)lds_tile_tag, (lds_u32*)lds_tile_cnt, tile_counts,
                           o.dense != 0};
