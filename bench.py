@@ -30,8 +30,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2048)
-    ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=4096)
+    ap.add_argument("--warmup", type=int, default=1024)
     ap.add_argument("--batch", type=int, default=64, help="frames per batched launch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU baseline sample")
@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--tile-split", type=int, default=0,
                     help="tile-list entries per raster workgroup before a tile is split "
                          "(0 keeps the library default)")
+    ap.add_argument("--pipelines", type=int, default=2, choices=[1, 2],
+                    help="context pipelines: 2 overlaps a draw's geometry kernels with the "
+                         "previous draw's raster (the library default)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the throughput lines of configs 3-5")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -252,7 +255,8 @@ def main():
             ctx.submit_batch(singles[k % n_variants])
 
     def set_tile(tw, th):
-        ctx.set_config(tile_w=tw, tile_h=th, tile_split=args.tile_split)
+        ctx.set_config(tile_w=tw, tile_h=th, tile_split=args.tile_split,
+                       pipelines=args.pipelines)
 
     tile_sweep = {}
     if args.tile == "auto":
@@ -274,7 +278,10 @@ def main():
     do_p = args.mode in ("both", "plain")
     steps = (max(args.steps, B) + B - 1) // B * B
     warm = (max(args.warmup, B) + B - 1) // B * B
-    # warm-up (also grows tile-list capacity if needed)
+    # warm-up (also grows tile-list capacity if needed); the batched form first,
+    # so that the context's buffers are allocated once at their largest size
+    if do_b:
+        run_batched(0, warm)
     if do_p:
         run_plain(0, max(1, warm // 4))
     if do_b:
@@ -293,8 +300,10 @@ def main():
     wall_b = timed(run_batched, steps) if do_b else float("nan")
     wall_p = timed(run_plain, plain_steps) if do_p else float("nan")
 
-    # per-kernel durations (HIP events around every launch, separate pass)
+    # per-kernel durations (HIP events around every launch, separate pass with
+    # one pipeline, so that no two kernels overlap inside a measured duration)
     kt_batched, kt_plain = {}, {}
+    ctx.set_config(pipelines=1)
     if do_b:
         ctx.reset_kernel_times()
         ctx.set_kernel_timing(True)
@@ -309,6 +318,7 @@ def main():
         ctx.synchronize()
         kt_plain = ctx.kernel_times()
         ctx.set_kernel_timing(False)
+    ctx.set_config(pipelines=args.pipelines)
 
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib << 20
@@ -413,7 +423,8 @@ def main():
                                "triangles, 16x16 RGBA8 texture, 1280x720, camera r=1.2",
                    "frames_per_batch": B, "triangles_per_frame": faces,
                    "tile": [ctx.config().tile_w, ctx.config().tile_h],
-                   "tile_split": ctx.config().tile_split},
+                   "tile_split": ctx.config().tile_split,
+                   "pipelines": ctx.config().pipelines},
         "shaded_mpix_s": round(shaded * main_fps / 1e6, 3),
         "covered_mfrag_s": round(covered * main_fps / 1e6, 3),
         "frames_per_s": {"batched": round(frames_s_b, 2), "plain": round(frames_s_p, 2),

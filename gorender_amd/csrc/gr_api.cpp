@@ -68,6 +68,19 @@ struct TimedLaunch {
     hipEvent_t a, b;
 };
 
+struct Pipe {
+    hipStream_t stream = nullptr;
+    // batch: frames | objects | stats | counters | frame_overflow | XCD tables,
+    // uploaded with one copy per draw (the counters as zeros).
+    DevBuf batch, rrec, srec, wrec, tile_counts, tiles, wgcount, refs, ctrl;
+    DevBuf items;  // raster work items (later parts of split tiles)
+    DevBuf slabs;  // depth keys of split tiles; zero between draws
+    DevBuf okey;   // overlay keys of the frames that draw overlays; zero between draws
+    DevBuf orec;   // overlay records per slot (batches with overlay frames)
+    hipEvent_t raster_done = nullptr;  // after this pipe's last k_raster
+    bool raster_pending = false;
+};
+
 }  // namespace
 
 
@@ -81,20 +94,22 @@ struct gr_context {
     DevBuf d_textures;
     bool tex_dirty = true;
 
-    // batch: frames | objects | stats | counters | frame_overflow,
-    // uploaded with one copy per draw (the counters as zeros).
-    DevBuf batch, rrec, srec, wrec, tile_counts, tiles, wgcount, refs, out_rgba, out_depth, ctrl;
-    DevBuf items;  // raster work items (later parts of split tiles)
-    DevBuf slabs;  // depth keys of split tiles; zero between draws
-    DevBuf okey;  // overlay keys of the frames that draw overlays; zero between draws
-    DevBuf orec;  // overlay records per slot (batches with overlay frames)
+    // Per-batch device state, twice: consecutive batches alternate between
+    // the two pipes (two streams), so one batch's face and binning kernels
+    // overlap the previous batch's raster. Each k_raster waits for the
+    // previous batch's, so frames are written in submission order.
+    Pipe pipes[2];
+    int last_pipe = -1;  // pipe of the last batch
+    bool one_pipe = false;  // synchronous draws: pipe 0 only
+    hipStream_t own1 = nullptr;  // pipe 1's stream
+    DevBuf out_rgba, out_depth;
     Staging ring[4];
     int ring_i = 0;
     void* h_ctrl = nullptr;   // pinned: sticky control words
     int32_t last_frames = 0;
     const unsigned long long* last_stats = nullptr;  // device stats of the last draw
 
-    gr_config cfg{32, 32, 1 << 22, 4, 512};
+    gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 0};
 
     bool timing = false;
     std::vector<TimedLaunch> timed;
@@ -110,11 +125,38 @@ int ilog2(int v) {
     return r;
 }
 
+gr_status sync_all(gr_context* c) {
+    GR_HIP(hipStreamSynchronize(c->stream));
+    for (Pipe& pp : c->pipes)
+        if (pp.stream && pp.stream != c->stream) GR_HIP(hipStreamSynchronize(pp.stream));
+    return GR_OK;
+}
+
+// Order work the caller queues on the context stream (fills, copies of its
+// buffers) with the draws: it starts after the last batch's raster, and the
+// next batch's raster starts after it.
+gr_status after_draws(gr_context* c) {
+    if (c->last_pipe >= 0 && c->pipes[c->last_pipe].raster_pending &&
+        c->pipes[c->last_pipe].stream != c->stream)
+        GR_HIP(hipStreamWaitEvent(c->stream, c->pipes[c->last_pipe].raster_done, 0));
+    return GR_OK;
+}
+gr_status before_draws(gr_context* c) {
+    Pipe& p0 = c->pipes[0];
+    p0.stream = c->stream;
+    if (!p0.raster_done) GR_HIP(hipEventCreateWithFlags(&p0.raster_done, hipEventDisableTiming));
+    GR_HIP(hipEventRecord(p0.raster_done, c->stream));
+    p0.raster_pending = true;
+    c->last_pipe = 0;
+    return GR_OK;
+}
+
 gr_status ensure(gr_context* c, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (b.cap >= bytes) return GR_OK;
     if (b.p) {
-        GR_HIP(hipStreamSynchronize(c->stream));
+        gr_status st = sync_all(c);
+        if (st) return st;
         GR_HIP(hipFree(b.p));
         b.p = nullptr;
         b.cap = 0;
@@ -147,21 +189,21 @@ gr_status staging_get(gr_context* c, size_t bytes, void** out, int* slot) {
     return GR_OK;
 }
 
-gr_status staging_mark(gr_context* c, int slot) {
-    GR_HIP(hipEventRecord(c->ring[slot].ev, c->stream));
+gr_status staging_mark(gr_context* c, int slot, hipStream_t s) {
+    GR_HIP(hipEventRecord(c->ring[slot].ev, s));
     c->ring[slot].pending = true;
     return GR_OK;
 }
 
-gr_status upload(gr_context* c, void* dst, const void* src, size_t bytes) {
+gr_status upload(gr_context* c, void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (bytes == 0) return GR_OK;
     void* h;
     int slot;
     gr_status st = staging_get(c, bytes, &h, &slot);
     if (st) return st;
     std::memcpy(h, src, bytes);
-    GR_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, c->stream));
-    return staging_mark(c, slot);
+    GR_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+    return staging_mark(c, slot, s);
 }
 
 hipEvent_t pool_event(gr_context* c) {
@@ -176,7 +218,7 @@ hipEvent_t pool_event(gr_context* c) {
 }
 
 template <typename F>
-void timed_launch(gr_context* c, const char* name, F f) {
+void timed_launch(gr_context* c, const char* name, hipStream_t s, F f) {
     if (!c->timing) {
         f();
         return;
@@ -186,9 +228,9 @@ void timed_launch(gr_context* c, const char* name, F f) {
         f();
         return;
     }
-    (void)hipEventRecord(a, c->stream);
+    (void)hipEventRecord(a, s);
     f();
-    (void)hipEventRecord(b, c->stream);
+    (void)hipEventRecord(b, s);
     c->timed.push_back({name, a, b});
 }
 
@@ -387,14 +429,24 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     }
     const int nobj = (int)dobj.size();
 
-    // Texture descriptor table.
+    // The pipe of this batch: the other one than the last batch's when the
+    // context runs on its own streams with two pipes, else pipe 0 on c->stream.
+    const bool two = c->stream == c->own && c->cfg.pipelines >= 2 && !c->one_pipe;
+    const int pi = two ? (c->last_pipe + 1) & 1 : 0;
+    Pipe& P = c->pipes[pi];
+    P.stream = pi == 0 ? c->stream : c->own1;
+    const hipStream_t s = P.stream;
+    gr_status st;
+
+    // Texture descriptor table (read by the raster of a batch that may still be
+    // running on the other pipe: wait for it before replacing the table).
     if (c->tex_dirty) {
         std::vector<gr::DevTexture> tt(std::max<size_t>(1, c->textures.size()));
         for (size_t i = 0; i < c->textures.size(); i++) tt[i] = c->textures[i].dt;
-        gr_status st = ensure(c, c->d_textures, tt.size() * sizeof(gr::DevTexture));
-        if (st) return st;
-        st = upload(c, c->d_textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture));
-        if (st) return st;
+        if ((st = sync_all(c))) return st;
+        if ((st = ensure(c, c->d_textures, tt.size() * sizeof(gr::DevTexture)))) return st;
+        if ((st = upload(c, c->d_textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture), s)))
+            return st;
         c->tex_dirty = false;
     }
 
@@ -412,20 +464,19 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     int face_grid = 0;
     gr::xcd_tables(nface.data(), nf, xcd.data(), &face_grid);
 
-    gr_status st;
-    if ((st = ensure(c, c->batch, batch_bytes))) return st;
-    if ((st = ensure(c, c->rrec, slots * sizeof(gr::RasterRec)))) return st;
-    if ((st = ensure(c, c->srec, slots * sizeof(gr::ShadeRec)))) return st;
-    if ((st = ensure(c, c->wrec, slots * sizeof(gr::WideRec)))) return st;
+    if ((st = ensure(c, P.batch, batch_bytes))) return st;
+    if ((st = ensure(c, P.rrec, slots * sizeof(gr::RasterRec)))) return st;
+    if ((st = ensure(c, P.srec, slots * sizeof(gr::ShadeRec)))) return st;
+    if ((st = ensure(c, P.wrec, slots * sizeof(gr::WideRec)))) return st;
     // Tile counts are zero between draws (k_raster re-zeroes the ones it used);
     // a new allocation starts zeroed.
-    if (c->tile_counts.cap < (size_t)tiles * 4) {
-        void* old = c->tile_counts.p;
-        if ((st = ensure(c, c->tile_counts, (size_t)tiles * 4))) return st;
-        if (c->tile_counts.p != old)
-            GR_HIP(hipMemsetAsync(c->tile_counts.p, 0, c->tile_counts.cap, c->stream));
+    if (P.tile_counts.cap < (size_t)tiles * 4) {
+        void* old = P.tile_counts.p;
+        if ((st = ensure(c, P.tile_counts, (size_t)tiles * 4))) return st;
+        if (P.tile_counts.p != old)
+            GR_HIP(hipMemsetAsync(P.tile_counts.p, 0, P.tile_counts.cap, s));
     }
-    if ((st = ensure(c, c->tiles, 3 * (size_t)tiles * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, P.tiles, 3 * (size_t)tiles * sizeof(uint32_t)))) return st;
     // Split tiles: a tile is split only when its list is longer than part_refs,
     // so the items of a batch number at most refs_cap / part_refs and the split
     // tiles at most refs_cap / (part_refs + 1); slabs are further capped by a
@@ -435,35 +486,35 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     const size_t slab_bytes = (size_t)TW * TH * 8;
     const uint32_t max_slabs = (uint32_t)std::max<unsigned long long>(
         1, std::min<unsigned long long>(refs_cap / (part_refs + 1ull) + 1, kSlabBudget / slab_bytes));
-    if ((st = ensure(c, c->items, (size_t)max_items * sizeof(gr::RasterItem)))) return st;
-    if (c->slabs.cap < (size_t)max_slabs * slab_bytes) {
-        void* old = c->slabs.p;
-        if ((st = ensure(c, c->slabs, (size_t)max_slabs * slab_bytes))) return st;
-        if (c->slabs.p != old) GR_HIP(hipMemsetAsync(c->slabs.p, 0, c->slabs.cap, c->stream));
+    if ((st = ensure(c, P.items, (size_t)max_items * sizeof(gr::RasterItem)))) return st;
+    if (P.slabs.cap < (size_t)max_slabs * slab_bytes) {
+        void* old = P.slabs.p;
+        if ((st = ensure(c, P.slabs, (size_t)max_slabs * slab_bytes))) return st;
+        if (P.slabs.p != old) GR_HIP(hipMemsetAsync(P.slabs.p, 0, P.slabs.cap, s));
     }
-    if ((st = ensure(c, c->wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
-    if ((st = ensure(c, c->refs, refs_cap * sizeof(uint32_t)))) return st;
-    if (!c->ctrl.p) {
-        if ((st = ensure(c, c->ctrl, sizeof(gr::DevControl)))) return st;
-        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
+    if ((st = ensure(c, P.wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, P.refs, refs_cap * sizeof(uint32_t)))) return st;
+    if (!P.ctrl.p) {
+        if ((st = ensure(c, P.ctrl, sizeof(gr::DevControl)))) return st;
+        GR_HIP(hipMemsetAsync(P.ctrl.p, 0, sizeof(gr::DevControl), s));
     }
     // Overlay keys are zero between draws (k_raster re-zeroes what it reads);
     // a new allocation starts zeroed.
-    if (okey_words && (st = ensure(c, c->orec, slots * sizeof(gr::OverlayRec)))) return st;
-    if (okey_words && c->okey.cap < okey_words * 4) {
-        void* old = c->okey.p;
-        if ((st = ensure(c, c->okey, okey_words * 4))) return st;
-        if (c->okey.p != old) GR_HIP(hipMemsetAsync(c->okey.p, 0, c->okey.cap, c->stream));
+    if (okey_words && (st = ensure(c, P.orec, slots * sizeof(gr::OverlayRec)))) return st;
+    if (okey_words && P.okey.cap < okey_words * 4) {
+        void* old = P.okey.p;
+        if ((st = ensure(c, P.okey, okey_words * 4))) return st;
+        if (P.okey.p != old) GR_HIP(hipMemsetAsync(P.okey.p, 0, P.okey.cap, s));
     }
 
     // Offsets -> device pointers, then one upload of the whole batch header.
     for (auto& d : frames) {
-        if (d.okey) d.okey = (uint32_t*)((char*)c->okey.p + ((uintptr_t)d.okey - 1));
-        if (d.orec) d.orec = (gr::OverlayRec*)((char*)c->orec.p + ((uintptr_t)d.orec - 1));
-        d.ctrl = (gr::DevControl*)c->ctrl.p;
-        d.rrec = (gr::RasterRec*)((char*)c->rrec.p + (uintptr_t)d.rrec);
-        d.srec = (gr::ShadeRec*)((char*)c->srec.p + (uintptr_t)d.srec);
-        d.wrec = (gr::WideRec*)((char*)c->wrec.p + (uintptr_t)d.wrec);
+        if (d.okey) d.okey = (uint32_t*)((char*)P.okey.p + ((uintptr_t)d.okey - 1));
+        if (d.orec) d.orec = (gr::OverlayRec*)((char*)P.orec.p + ((uintptr_t)d.orec - 1));
+        d.ctrl = (gr::DevControl*)P.ctrl.p;
+        d.rrec = (gr::RasterRec*)((char*)P.rrec.p + (uintptr_t)d.rrec);
+        d.srec = (gr::ShadeRec*)((char*)P.srec.p + (uintptr_t)d.srec);
+        d.wrec = (gr::WideRec*)((char*)P.wrec.p + (uintptr_t)d.wrec);
     }
     {
         void* h;
@@ -473,10 +524,10 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         std::memcpy(h, frames.data(), (size_t)nf * sizeof(gr::DevFrame));
         if (nobj) std::memcpy((char*)h + off_objs, dobj.data(), (size_t)nobj * sizeof(gr::DevObject));
         std::memcpy((char*)h + off_xcd, xcd.data(), xcd.size() * sizeof(int));
-        GR_HIP(hipMemcpyAsync(c->batch.p, h, batch_bytes, hipMemcpyHostToDevice, c->stream));
-        if ((st = staging_mark(c, slot))) return st;
+        GR_HIP(hipMemcpyAsync(P.batch.p, h, batch_bytes, hipMemcpyHostToDevice, s));
+        if ((st = staging_mark(c, slot, s))) return st;
     }
-    char* bb = (char*)c->batch.p;
+    char* bb = (char*)P.batch.p;
 
     gr::LaunchParams p{};
     p.objs = (const gr::DevObject*)(bb + off_objs);
@@ -488,39 +539,47 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     p.total_tiles = tiles;
     p.tw_log2 = twl;
     p.th_log2 = thl;
-    p.tile_counts = (uint32_t*)c->tile_counts.p;
-    p.tile_offsets = (uint32_t*)c->tiles.p;
-    p.tile_cursor = (uint32_t*)c->tiles.p + tiles;
-    p.tile_slab = (uint32_t*)c->tiles.p + 2 * (size_t)tiles;
-    p.items = (gr::RasterItem*)c->items.p;
+    p.tile_counts = (uint32_t*)P.tile_counts.p;
+    p.tile_offsets = (uint32_t*)P.tiles.p;
+    p.tile_cursor = (uint32_t*)P.tiles.p + tiles;
+    p.tile_slab = (uint32_t*)P.tiles.p + 2 * (size_t)tiles;
+    p.items = (gr::RasterItem*)P.items.p;
     p.max_items = max_items;
-    p.slabs = (unsigned long long*)c->slabs.p;
+    p.slabs = (unsigned long long*)P.slabs.p;
     p.max_slabs = max_slabs;
     p.part_refs = part_refs;
     p.frame_overflow = (uint32_t*)(bb + off_over);
-    p.wg_count = (uint32_t*)c->wgcount.p;
-    p.refs = (uint32_t*)c->refs.p;
+    p.wg_count = (uint32_t*)P.wgcount.p;
+    p.refs = (uint32_t*)P.refs.p;
     p.refs_capacity = refs_cap;
     p.stats = (unsigned long long*)(bb + off_stats);
     p.counters = (gr::DevBatchCounters*)(bb + off_counters);
-    p.ctrl = (gr::DevControl*)c->ctrl.p;
+    p.ctrl = (gr::DevControl*)P.ctrl.p;
     p.xcd_face = (const int*)(bb + off_xcd);
     p.face_grid = face_grid;
 
-    timed_launch(c, "k_face", [&] { gr::launch_face(p, c->stream); });
-    timed_launch(c, "k_bin_scan", [&] { gr::launch_bin_scan(p, c->stream); });
-    timed_launch(c, "k_bin_fill", [&] { gr::launch_bin_fill(p, c->stream); });
-    if (okey_words) timed_launch(c, "k_overlay", [&] { gr::launch_overlay(p, c->stream); });
+    timed_launch(c, "k_face", s, [&] { gr::launch_face(p, s); });
+    timed_launch(c, "k_bin_scan", s, [&] { gr::launch_bin_scan(p, s); });
+    timed_launch(c, "k_bin_fill", s, [&] { gr::launch_bin_fill(p, s); });
+    if (okey_words) timed_launch(c, "k_overlay", s, [&] { gr::launch_overlay(p, s); });
     bool ok = true;
-    timed_launch(c, "k_raster_parts", [&] { ok = gr::launch_raster_parts(p, c->stream); });
-    if (ok) timed_launch(c, "k_raster", [&] { ok = gr::launch_raster(p, c->stream); });
+    timed_launch(c, "k_raster_parts", s, [&] { ok = gr::launch_raster_parts(p, s); });
+    // Frames are written in submission order: this raster starts after the
+    // previous batch's (on the other pipe).
+    if (c->last_pipe >= 0 && c->last_pipe != pi && c->pipes[c->last_pipe].raster_pending)
+        GR_HIP(hipStreamWaitEvent(s, c->pipes[c->last_pipe].raster_done, 0));
+    if (ok) timed_launch(c, "k_raster", s, [&] { ok = gr::launch_raster(p, s); });
     hipError_t le = hipGetLastError();
     if (!ok || le != hipSuccess) {
         // Leave the tile counters zeroed for the next draw.
-        (void)hipMemsetAsync(c->tile_counts.p, 0, c->tile_counts.cap, c->stream);
+        (void)hipMemsetAsync(P.tile_counts.p, 0, P.tile_counts.cap, s);
         if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
         return fail(GR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(le));
     }
+    if (!P.raster_done) GR_HIP(hipEventCreateWithFlags(&P.raster_done, hipEventDisableTiming));
+    GR_HIP(hipEventRecord(P.raster_done, s));
+    P.raster_pending = true;
+    c->last_pipe = pi;
     c->last_frames = nf;
     c->last_stats = p.stats;
     return GR_OK;
@@ -530,7 +589,8 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
 gr_status read_stats(gr_context* c, std::vector<unsigned long long>& out) {
     out.assign((size_t)c->last_frames * gr::kNumStats, 0ull);
     if (!c->last_stats || out.empty()) return GR_OK;
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     GR_HIP(hipMemcpy(out.data(), c->last_stats, out.size() * 8, hipMemcpyDeviceToHost));
     return GR_OK;
 }
@@ -546,26 +606,29 @@ void to_gr_stats(const unsigned long long* q, gr_stats* s) {
     s->tiles_split = (int64_t)q[gr::kStatSplit];
 }
 
-// Wait, read and clear the sticky overflow words.
+// Wait for every pipe, then read and clear the sticky control words of both.
 gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
-    GR_HIP(hipMemcpyAsync(c->h_ctrl, c->ctrl.p, sizeof(gr::DevControl), hipMemcpyDeviceToHost,
-                          c->stream));
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     if (c->timing) collect_times(c);
-    gr::DevControl ctl;
-    std::memcpy(&ctl, c->h_ctrl, sizeof ctl);
     *needed = 0;
-    if (ctl.unsupported) {
-        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
-        GR_HIP(hipStreamSynchronize(c->stream));
-        return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+    bool unsupported = false, overflow = false;
+    for (Pipe& pp : c->pipes) {
+        if (!pp.ctrl.p) continue;
+        gr::DevControl ctl;
+        GR_HIP(hipMemcpyAsync(c->h_ctrl, pp.ctrl.p, sizeof ctl, hipMemcpyDeviceToHost, pp.stream));
+        GR_HIP(hipStreamSynchronize(pp.stream));
+        std::memcpy(&ctl, c->h_ctrl, sizeof ctl);
+        if (ctl.unsupported || ctl.overflow) {
+            unsupported = unsupported || ctl.unsupported;
+            overflow = overflow || ctl.overflow;
+            *needed = std::max(*needed, ctl.refs_needed);
+            GR_HIP(hipMemsetAsync(pp.ctrl.p, 0, sizeof ctl, pp.stream));
+            GR_HIP(hipStreamSynchronize(pp.stream));
+        }
     }
-    if (ctl.overflow) {
-        *needed = ctl.refs_needed;
-        GR_HIP(hipMemsetAsync(c->ctrl.p, 0, sizeof(gr::DevControl), c->stream));
-        GR_HIP(hipStreamSynchronize(c->stream));
-        return GR_ERR_CAPACITY;
-    }
+    if (unsupported) return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+    if (overflow) return GR_ERR_CAPACITY;
     return GR_OK;
 }
 
@@ -611,14 +674,17 @@ gr_status enqueue_batch(gr_context* c, const gr_batch_frame* frames, int32_t nf,
         return ::fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
     }
 }
-hipStream_t context_stream(gr_context* c) { return c->stream; }
+hipStream_t context_stream(gr_context* c) {
+    return c->last_pipe >= 0 && c->pipes[c->last_pipe].stream ? c->pipes[c->last_pipe].stream
+                                                              : c->stream;
+}
 int context_device(gr_context* c) { return c->device; }
 const void* last_stats_device(gr_context* c) { return c->last_stats; }
 size_t stats_bytes_per_frame() { return gr::kNumStats * sizeof(unsigned long long); }
 void decode_stats(const void* frame_stats, gr_stats* out) {
     to_gr_stats((const unsigned long long*)frame_stats, out);
 }
-void* control_device(gr_context* c) { return c->ctrl.p; }
+void* control_device(gr_context* c) { return c->last_pipe >= 0 ? c->pipes[c->last_pipe].ctrl.p : nullptr; }
 size_t control_bytes() { return sizeof(gr::DevControl); }
 void decode_control(const void* host_copy, bool* overflow, unsigned long long* refs_needed,
                     bool* unsupported) {
@@ -629,6 +695,7 @@ void decode_control(const void* host_copy, bool* overflow, unsigned long long* r
     *unsupported = ctl.unsupported != 0;
 }
 void grow_refs_to(gr_context* c, unsigned long long needed) { grow_refs(c, needed); }
+void set_one_pipe(gr_context* c, bool on) { c->one_pipe = on; }
 bool mesh_live(gr_context* c, int32_t h) {
     return h >= 0 && h < (int32_t)c->meshes.size() && c->meshes[h].live;
 }
@@ -655,7 +722,13 @@ gr_status gr_create(int32_t device, gr_context** out) {
             return fail(GR_ERR_DEVICE, "hipStreamCreate failed");
         }
         c->stream = c->own;
+        if (hipStreamCreateWithFlags(&c->own1, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipStreamDestroy(c->own);
+            delete c;
+            return fail(GR_ERR_DEVICE, "hipStreamCreate failed");
+        }
         if (hipHostMalloc(&c->h_ctrl, sizeof(gr::DevControl), hipHostMallocDefault) != hipSuccess) {
+            (void)hipStreamDestroy(c->own1);
             (void)hipStreamDestroy(c->own);
             delete c;
             return fail(GR_ERR_OUT_OF_MEMORY, "pinned allocation failed");
@@ -676,11 +749,19 @@ void gr_destroy(gr_context* c) {
     for (auto& m : c->meshes) free_mesh(m);
     for (auto& t : c->textures)
         if (t.texels) (void)hipFree(t.texels);
-    DevBuf* bufs[] = {&c->d_textures, &c->batch, &c->rrec, &c->srec, &c->wrec,
-                      &c->tile_counts, &c->tiles, &c->wgcount, &c->refs, &c->out_rgba, &c->okey, &c->orec,
-                      &c->out_depth, &c->ctrl, &c->items, &c->slabs};
+    (void)sync_all(c);
+    DevBuf* bufs[] = {&c->d_textures, &c->out_rgba, &c->out_depth};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
+    for (Pipe& pp : c->pipes) {
+        DevBuf* pb[] = {&pp.batch, &pp.rrec,    &pp.srec,  &pp.wrec,  &pp.tile_counts,
+                        &pp.tiles, &pp.wgcount, &pp.refs,  &pp.ctrl,  &pp.items,
+                        &pp.slabs, &pp.okey,    &pp.orec};
+        for (DevBuf* b : pb)
+            if (b->p) (void)hipFree(b->p);
+        if (pp.raster_done) (void)hipEventDestroy(pp.raster_done);
+    }
+    if (c->own1) (void)hipStreamDestroy(c->own1);
     for (auto& s : c->ring) {
         if (s.host) (void)hipHostFree(s.host);
         if (s.ev) (void)hipEventDestroy(s.ev);
@@ -703,9 +784,12 @@ gr_status gr_set_config(gr_context* c, const gr_config* cfg) {
     if (cfg->refs_capacity) n.refs_capacity = cfg->refs_capacity;
     if (cfg->raster_waves) n.raster_waves = cfg->raster_waves;
     if (cfg->tile_split) n.tile_split = cfg->tile_split;
+    if (cfg->pipelines) n.pipelines = cfg->pipelines;
     if (!valid_tile(n.tile_w) || !valid_tile(n.tile_h))
         return fail(GR_ERR_INVALID_ARGUMENT, "tile sides must be 8, 16, 32 or 64");
     if (n.raster_waves != 4) return fail(GR_ERR_INVALID_ARGUMENT, "raster_waves must be 4");
+    if (n.pipelines != 1 && n.pipelines != 2)
+        return fail(GR_ERR_INVALID_ARGUMENT, "pipelines must be 1 or 2");
     if (n.refs_capacity < 1024) return fail(GR_ERR_INVALID_ARGUMENT, "refs_capacity too small");
     if (n.tile_split < 64 || n.tile_split > (1 << 24))
         return fail(GR_ERR_INVALID_ARGUMENT, "tile_split must be in [64, 2^24]");
@@ -721,7 +805,8 @@ gr_status gr_get_config(gr_context* c, gr_config* cfg) {
 
 gr_status gr_set_stream(gr_context* c, void* s) {
     if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     c->stream = s ? (hipStream_t)s : c->own;
     return GR_OK;
 }
@@ -779,7 +864,8 @@ gr_status gr_texture_create(gr_context* c, const gr_texture_desc* d, const uint8
 gr_status gr_texture_destroy(gr_context* c, int32_t h) {
     if (!c || h < 0 || h >= (int32_t)c->textures.size() || !c->textures[h].live)
         return fail(GR_ERR_INVALID_HANDLE, "invalid texture handle");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     if (c->textures[h].texels) GR_HIP(hipFree(c->textures[h].texels));
     c->textures[h] = TexRes{};
     c->tex_dirty = true;
@@ -962,7 +1048,8 @@ gr_status gr_mesh_create(gr_context* c, const gr_mesh_desc* d, int32_t* out) {
 gr_status gr_mesh_destroy(gr_context* c, int32_t h) {
     if (!c || h < 0 || h >= (int32_t)c->meshes.size() || !c->meshes[h].live)
         return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     free_mesh(c->meshes[h]);
     c->meshes[h] = MeshRes{};
     return GR_OK;
@@ -1031,10 +1118,7 @@ gr_status gr_draw_device(gr_context* c, const gr_frame* frame, const gr_object* 
 
 gr_status gr_synchronize(gr_context* c) {
     if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
-    if (!c->h_ctrl || !c->ctrl.p) {
-        GR_HIP(hipStreamSynchronize(c->stream));
-        return GR_OK;
-    }
+    if (!c->h_ctrl) return sync_all(c);
     unsigned long long needed = 0;
     gr_status st = sync_and_check(c, &needed);
     if (st == GR_ERR_CAPACITY) {
@@ -1054,6 +1138,12 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
         size_t npx = (size_t)frame->width * frame->height;
         if ((st = ensure(c, c->out_rgba, npx * 4))) return st;
         if ((st = ensure(c, c->out_depth, npx * 4))) return st;
+        // Synchronous: each frame is waited for, so there is nothing to overlap.
+        struct OnePipe {
+            gr_context* c;
+            ~OnePipe() { c->one_pipe = false; }
+        } one{c};
+        c->one_pipe = true;
         for (int attempt = 0; attempt < 4; attempt++) {
             st = gr_draw_device(c, frame, objs, no, c->out_rgba.p, c->out_depth.p);
             if (st) return st;
@@ -1101,23 +1191,29 @@ gr_status gr_device_alloc(gr_context* c, size_t bytes, void** out) {
 gr_status gr_device_free(gr_context* c, void* p) {
     if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
     if (!p) return GR_OK;
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     GR_HIP(hipFree(p));
     return GR_OK;
 }
 
 gr_status gr_device_fill(gr_context* c, void* p, int32_t value, size_t bytes) {
     if (!c || (!p && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
-    if (bytes) GR_HIP(hipMemsetAsync(p, value, bytes, c->stream));
-    return GR_OK;
+    if (!bytes) return GR_OK;
+    gr_status st = after_draws(c);
+    if (st) return st;
+    GR_HIP(hipMemsetAsync(p, value, bytes, c->stream));
+    return before_draws(c);
 }
 
 gr_status gr_device_copy(gr_context* c, void* dst, const void* src, size_t bytes) {
     if (!c || ((!dst || !src) && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
     GR_HIP(hipSetDevice(c->device));
-    timed_launch(c, "k_copy", [&] { gr::launch_copy(dst, src, bytes, c->stream); });
+    gr_status st = after_draws(c);
+    if (st) return st;
+    timed_launch(c, "k_copy", c->stream, [&] { gr::launch_copy(dst, src, bytes, c->stream); });
     GR_HIP(hipGetLastError());
-    return GR_OK;
+    return before_draws(c);
 }
 
 gr_status gr_check_exact_division(gr_context* c, uint64_t count, uint64_t seed,
@@ -1145,14 +1241,16 @@ gr_status gr_check_exact_division(gr_context* c, uint64_t count, uint64_t seed,
 
 gr_status gr_copy_to_host(gr_context* c, void* dst, const void* src, size_t bytes) {
     if (!c || ((!dst || !src) && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return GR_OK;
 }
 
 gr_status gr_set_kernel_timing(gr_context* c, int32_t enabled) {
     if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
-    GR_HIP(hipStreamSynchronize(c->stream));
+    gr_status st = sync_all(c);
+    if (st) return st;
     collect_times(c);
     c->timing = enabled != 0;
     return GR_OK;
@@ -1161,7 +1259,7 @@ gr_status gr_set_kernel_timing(gr_context* c, int32_t enabled) {
 int32_t gr_kernel_times(gr_context* c, const char** names, double* total_ms, int64_t* launches,
                         int32_t max) {
     if (!c) return 0;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     collect_times(c);
     int32_t i = 0;
     static std::mutex mu;
@@ -1180,7 +1278,7 @@ int32_t gr_kernel_times(gr_context* c, const char** names, double* total_ms, int
 
 void gr_reset_kernel_times(gr_context* c) {
     if (!c) return;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     collect_times(c);
     c->kernel_ms.clear();
 }

@@ -27,5 +27,7 @@ void decode_control(const void* host_copy, bool* overflow, unsigned long long* r
 // Tile-list capacity for the next batch, after an overflow that needed `needed`.
 void grow_refs_to(gr_context* c, unsigned long long needed);
 bool mesh_live(gr_context* c, int32_t handle);
+// Keep every batch on pipe 0 (the serving loop overlaps its own readback).
+void set_one_pipe(gr_context* c, bool on);
 
 }  // namespace gr_internal

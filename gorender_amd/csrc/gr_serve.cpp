@@ -353,6 +353,7 @@ gr_status gr_server_create(gr_context* ctx, const gr_server_config* cfg, gr_serv
             hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
             return fail(GR_ERR_DEVICE, "server copy stream");
         gr_server* raw = s.get();
+        gr_internal::set_one_pipe(ctx, true);
         s->worker = std::thread([raw] { raw->run(); });
         *out = s.release();
         return GR_OK;
@@ -369,6 +370,7 @@ void gr_server_destroy(gr_server* s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    gr_internal::set_one_pipe(s->ctx, false);
     delete s;
 }
 

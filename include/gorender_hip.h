@@ -159,6 +159,12 @@ typedef struct {
     int32_t tile_split;           /* tile-list entries per raster workgroup, 64..2^24: a
                                      longer list is split into parts that several
                                      workgroups resolve (same result, more parallelism) */
+    int32_t pipelines;            /* 1 or 2 (default): with 2, consecutive asynchronous draws
+                                     alternate between two sets of buffers and streams, so a
+                                     draw's geometry stages overlap the previous draw's raster;
+                                     frames are still written in submission order. A stream
+                                     set with gr_set_stream runs everything on it (1). */
+    int32_t reserved;
 } gr_config;
 
 typedef struct gr_context gr_context;
