@@ -166,13 +166,13 @@ def serving(ctx, requests, mh):
     return res
 
 
-def profile_config5(steps, warmup, tile_split):
+def profile_config5(steps, warmup, tile_split, pipelines):
     """The config-5 batch (8 requests over the 32 resident meshes) launched
     `steps` times, for kernel traces of the split-tile path."""
     from gorender_amd import scenes
     from gorender_amd.device import Context
     ctx = Context(0)
-    ctx.set_config(tile_split=tile_split)
+    ctx.set_config(tile_split=tile_split, pipelines=pipelines)
     rs = scenes.config5(8)
     th = [ctx.upload_texture(t) for t in rs.textures]
     mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
@@ -212,7 +212,7 @@ def main():
     from gorender_amd.device import Context
 
     if args.mode == "config5":
-        profile_config5(args.steps, args.warmup, args.tile_split)
+        profile_config5(args.steps, args.warmup, args.tile_split, args.pipelines)
         return
 
     scene = scenes.config2()
