@@ -304,6 +304,12 @@ def main():
     # one pipeline, so that no two kernels overlap inside a measured duration)
     kt_batched, kt_plain = {}, {}
     ctx.set_config(pipelines=1)
+    # plain form strictly one frame after another (one pipeline: no two
+    # frames' kernels overlap)
+    wall_p1 = float("nan")
+    if do_p:
+        run_plain(0, 64)
+        wall_p1 = timed(run_plain, plain_steps)
     if do_b:
         ctx.reset_kernel_times()
         ctx.set_kernel_timing(True)
@@ -351,6 +357,7 @@ def main():
     # against the oracle is done by tests; here record the counters only.
     frames_s_b = steps / wall_b
     frames_s_p = plain_steps / wall_p
+    frames_s_p1 = plain_steps / wall_p1
     mtris_b = faces * frames_s_b / 1e6
     mtris_p = faces * frames_s_p / 1e6
     # the reported value is the batched form (the plain form when run alone)
@@ -428,7 +435,9 @@ def main():
         "shaded_mpix_s": round(shaded * main_fps / 1e6, 3),
         "covered_mfrag_s": round(covered * main_fps / 1e6, 3),
         "frames_per_s": {"batched": round(frames_s_b, 2), "plain": round(frames_s_p, 2),
-                         "batched_over_plain": round(frames_s_b / frames_s_p, 3)},
+                         "plain_one_pipeline": round(frames_s_p1, 2),
+                         "batched_over_plain": round(frames_s_b / frames_s_p, 3),
+                         "batched_over_plain_one_pipeline": round(frames_s_b / frames_s_p1, 3)},
         "mtris_s": {"batched": round(mtris_b, 3), "plain": round(mtris_p, 3)},
         "latency_ms": ({"device_p50": round(pct(lat_dev, 50), 4),
                         "device_p99": round(pct(lat_dev, 99), 4),
