@@ -280,13 +280,22 @@ def main():
     warm = (max(args.warmup, B) + B - 1) // B * B
     # warm-up (also grows tile-list capacity if needed); the batched form first,
     # so that the context's buffers are allocated once at their largest size
-    if do_b:
-        run_batched(0, warm)
-    if do_p:
-        run_plain(0, max(1, warm // 4))
-    if do_b:
-        run_batched(0, warm)
-    ctx.synchronize()
+    def warm_up():
+        if do_b:
+            run_batched(0, warm)
+        if do_p:
+            run_plain(0, max(1, warm // 4))
+        if do_b:
+            run_batched(0, warm)
+        ctx.synchronize()
+
+    from gorender_amd import _lib
+    try:
+        warm_up()
+    except _lib.GorenderError as e:
+        if e.status != _lib.GR_ERR_CAPACITY:
+            raise
+        warm_up()  # the library has grown the tile-list capacity: draw again
     stats = ctx.last_stats()[0]
 
     def timed(fn, nframes):

@@ -334,6 +334,36 @@ bool dense_on_screen(const float* mvp, const float bbox[8][4], int F, int W, int
 // Device memory for the depth keys of split tiles.
 constexpr unsigned long long kSlabBudget = 64ull << 20;
 
+// Everything that makes one frame of a batch invalid, checked before any
+// launch. The serving loop runs it when a request is submitted, so one bad
+// request cannot fail a batch it would share.
+gr_status check_frame(gr_context* c, const gr_frame& fr, const gr_object* objs, int n) {
+    gr_status st = validate_frame(fr);
+    if (st) return st;
+    size_t blocks = 0;
+    for (int k = 0; k < n; k++) {
+        const gr_object& ob = objs[k];
+        if (ob.mesh < 0 || ob.mesh >= (int)c->meshes.size() || !c->meshes[ob.mesh].live)
+            return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
+        const MeshRes& mr = c->meshes[ob.mesh];
+        for (int32_t th : mr.textures)
+            if (th < 0 || th >= (int)c->textures.size() || !c->textures[th].live)
+                return fail(GR_ERR_INVALID_HANDLE, "mesh references a destroyed texture");
+        const bool gouraud =
+            (fr.flags & GR_LIGHTING) && !(fr.flags & GR_FLAT_SHADING) && mr.dm.N > 0;
+        if (gouraud && !mr.ni_valid)
+            return fail(GR_ERR_INDEX_OUT_OF_RANGE,
+                        "normal index out of range under Gouraud shading");
+        blocks += (size_t)((mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg);
+    }
+    if (blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
+        return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
+    if ((fr.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES)) &&
+        blocks * gr::kSlotsPerWg >= 0x7FFFFFFFull)
+        return fail(GR_ERR_UNSUPPORTED, "overlays on a frame with more than 2^31 slots");
+    return GR_OK;
+}
+
 // The whole pipeline for a batch of frames. Objects must be laid out frame by
 // frame; outputs are device pointers.
 gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
@@ -349,11 +379,11 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     size_t slots = 0, okey_words = 0;
     for (int f = 0; f < nf; f++) {
         const gr_frame& fr = bf[f].frame;
-        gr_status st = validate_frame(fr);
-        if (st) return st;
         if (bf[f].num_objects < 0 || bf[f].first_object < 0 ||
             bf[f].first_object + bf[f].num_objects > no)
             return fail(GR_ERR_INVALID_ARGUMENT, "frame object range out of bounds");
+        gr_status st = check_frame(c, fr, objs + bf[f].first_object, bf[f].num_objects);
+        if (st) return st;
         if (!bf[f].d_rgba || !bf[f].d_depth)
             return fail(GR_ERR_INVALID_ARGUMENT, "null device output");
         gr::DevFrame& d = frames[f];
@@ -392,12 +422,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         d.obj_count = bf[f].num_objects;
         for (int k = 0; k < bf[f].num_objects; k++) {
             const gr_object& ob = objs[bf[f].first_object + k];
-            if (ob.mesh < 0 || ob.mesh >= (int)c->meshes.size() || !c->meshes[ob.mesh].live)
-                return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
             const MeshRes& mr = c->meshes[ob.mesh];
-            for (int32_t th : mr.textures)
-                if (th < 0 || th >= (int)c->textures.size() || !c->textures[th].live)
-                    return fail(GR_ERR_INVALID_HANDLE, "mesh references a destroyed texture");
             gr::DevObject o;
             std::memset(&o, 0, sizeof o);
             o.m = mr.dm;
@@ -405,9 +430,6 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             std::memcpy(o.world, ob.world, sizeof o.world);
             o.frame = f;
             o.gouraud = (fr.flags & GR_LIGHTING) && !(fr.flags & GR_FLAT_SHADING) && mr.dm.N > 0;
-            if (o.gouraud && !mr.ni_valid)
-                return fail(GR_ERR_INDEX_OUT_OF_RANGE,
-                            "normal index out of range under Gouraud shading");
             o.dense = dense_on_screen(ob.mvp, mr.dm.bbox, mr.dm.F, fr.width, fr.height) ? 1 : 0;
             o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
             o.face_block = fblocks;
@@ -417,10 +439,6 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             dobj.push_back(o);
         }
         d.face_blocks = fblocks - d.face_block_begin;
-        if ((size_t)d.face_blocks * gr::kSlotsPerWg >= 0xFFFFFFFFull)
-            return fail(GR_ERR_INVALID_ARGUMENT, "too many faces in one frame");
-        if (d.okey && (size_t)d.face_blocks * gr::kSlotsPerWg >= 0x7FFFFFFFull)
-            return fail(GR_ERR_UNSUPPORTED, "overlays on a frame with more than 2^31 slots");
         d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)(slots * sizeof(gr::OverlayRec) + 1) : nullptr;
         d.rrec = (gr::RasterRec*)(uintptr_t)(slots * sizeof(gr::RasterRec));
         d.srec = (gr::ShadeRec*)(uintptr_t)(slots * sizeof(gr::ShadeRec));
@@ -696,8 +714,9 @@ void decode_control(const void* host_copy, bool* overflow, unsigned long long* r
 }
 void grow_refs_to(gr_context* c, unsigned long long needed) { grow_refs(c, needed); }
 void set_one_pipe(gr_context* c, bool on) { c->one_pipe = on; }
-bool mesh_live(gr_context* c, int32_t h) {
-    return h >= 0 && h < (int32_t)c->meshes.size() && c->meshes[h].live;
+gr_status check_request(gr_context* c, const gr_frame* frame, const gr_object* objects,
+                        int32_t num_objects) {
+    return check_frame(c, *frame, objects, num_objects);
 }
 
 }  // namespace gr_internal

@@ -26,7 +26,11 @@ void decode_control(const void* host_copy, bool* overflow, unsigned long long* r
                     bool* unsupported);
 // Tile-list capacity for the next batch, after an overflow that needed `needed`.
 void grow_refs_to(gr_context* c, unsigned long long needed);
-bool mesh_live(gr_context* c, int32_t handle);
+// The checks a frame must pass to be drawn (frame fields, mesh and texture
+// handles, normal indices under Gouraud shading, slot bounds); the status
+// and message a batch holding it would fail with.
+gr_status check_request(gr_context* c, const gr_frame* frame, const gr_object* objects,
+                        int32_t num_objects);
 // Keep every batch on pipe 0 (the serving loop overlaps its own readback).
 void set_one_pipe(gr_context* c, bool on);
 

@@ -378,13 +378,12 @@ gr_status gr_server_submit(gr_server* s, const gr_frame* frame, const gr_object*
                            int32_t num_objects, int64_t* ticket) {
     if (!s || !frame || !ticket || num_objects < 0 || (num_objects > 0 && !objects))
         return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
-    if (frame->width < 1 || frame->height < 1 || frame->width > 16384 || frame->height > 16384)
-        return fail(GR_ERR_INVALID_ARGUMENT, "frame size out of range");
+    // The checks a batch would make of this frame, so that a bad request is
+    // refused alone instead of failing the batch it would share.
+    const gr_status cst = gr_internal::check_request(s->ctx, frame, objects, num_objects);
+    if (cst) return cst;
     if (target_bytes(*frame) > (size_t)s->cfg.target_budget)
         return fail(GR_ERR_INVALID_ARGUMENT, "frame larger than the server's target budget");
-    for (int32_t k = 0; k < num_objects; k++)
-        if (!gr_internal::mesh_live(s->ctx, objects[k].mesh))
-            return fail(GR_ERR_INVALID_HANDLE, "invalid mesh handle");
     try {
         std::unique_ptr<Request> r(new Request);
         r->frame = *frame;

@@ -97,3 +97,35 @@ def test_open_loop_replay(ctx, resident):
     assert bs.min() >= 1 and bs.max() > 1
     print(f"replay p50 {np.percentile(lat, 50):.3f} ms p99 {np.percentile(lat, 99):.3f} ms "
           f"mean batch {bs.mean():.2f}", flush=True)
+
+
+def test_bad_requests_refused_alone(ctx, resident):
+    """Requests that a batch would refuse (a reference-tile count other than 1
+    or 16; Gouraud lighting over normal indices out of range) are refused at
+    submit, so they cannot fail a shared batch: the server keeps serving the
+    others, each frame equal to the oracle's."""
+    rs, reqs = resident
+    v = np.array([[0, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1]], np.float32)
+    m = S.MeshData(v, np.array([[0, 1, 2]], np.int32), v.copy(),
+                   np.array([[0, 1, -1]], np.int32))
+    h = ctx.upload_mesh(m)  # before the server owns the context
+    fr = S.FrameData(32, 32)
+    lit = PreparedRequest(
+        S.SceneData(fr, [m], [], [S.camera_object(fr, 0, (0, 0, 3), (0, 0, -1))]), [h])
+    ok = reqs[0]
+    tiles = ok.frame.ref_tiles
+    with Server(ctx, 1 << 30, 8, 2) as srv:
+        ok.frame.ref_tiles = 3
+        try:
+            with pytest.raises(_lib.GorenderError) as e:
+                srv.submit(ok)
+        finally:
+            ok.frame.ref_tiles = tiles
+        assert e.value.status == _lib.GR_ERR_INVALID_ARGUMENT
+        with pytest.raises(_lib.GorenderError) as e:
+            srv.submit(lit)
+        assert e.value.status == _lib.GR_ERR_INDEX_OUT_OF_RANGE
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    for k, res in enumerate(results):
+        check_result(k, res)
