@@ -702,6 +702,9 @@ size_t stats_bytes_per_frame() { return gr::kNumStats * sizeof(unsigned long lon
 void decode_stats(const void* frame_stats, gr_stats* out) {
     to_gr_stats((const unsigned long long*)frame_stats, out);
 }
+bool frame_long_lines(const void* frame_stats) {
+    return ((const unsigned long long*)frame_stats)[gr::kStatLongLines] != 0;
+}
 void* control_device(gr_context* c) { return c->last_pipe >= 0 ? c->pipes[c->last_pipe].ctrl.p : nullptr; }
 size_t control_bytes() { return sizeof(gr::DevControl); }
 void decode_control(const void* host_copy, bool* overflow, unsigned long long* refs_needed,

@@ -146,6 +146,7 @@ enum : int32_t {
     kStatClipOverflow,
     kStatRefs,
     kStatSplit,  // tiles whose list was split across raster workgroups
+    kStatLongLines,  // overlay lines too long to draw (not in gr_stats)
     kNumStats
 };
 

@@ -19,6 +19,8 @@ int context_device(gr_context* c);
 const void* last_stats_device(gr_context* c);
 size_t stats_bytes_per_frame();
 void decode_stats(const void* frame_stats, gr_stats* out);
+// Whether the frame had an overlay line too long to draw (GR_ERR_UNSUPPORTED).
+bool frame_long_lines(const void* frame_stats);
 // Sticky control words (device) and their size; decode a host copy of them.
 void* control_device(gr_context* c);
 size_t control_bytes();

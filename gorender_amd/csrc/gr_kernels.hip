@@ -428,14 +428,15 @@ __device__ void ov_rect(const DevFrame& fr, int64_t x, int64_t y, uint32_t code)
 }
 
 // FrameBuffer.Line (rasterizer.go:65-79): a float32 DDA accumulated step by step.
-__device__ void ov_line(const DevFrame& fr, int64_t x0, int64_t y0, int64_t x1, int64_t y1,
+// Returns false, drawing nothing, for a line longer than kMaxOverlayLine.
+__device__ bool ov_line(const DevFrame& fr, int64_t x0, int64_t y0, int64_t x1, int64_t y1,
                         uint32_t code) {
     const int64_t dx = wsub(x1, x0), dy = wsub(y1, y0);
     const int64_t ax = go_abs(dx), ay = go_abs(dy);
     const int64_t side = ax > ay ? ax : ay;
     if (side > kMaxOverlayLine) {
         atomicOr(&fr.ctrl->unsupported, 1u);
-        return;
+        return false;
     }
     const float xs = (float)dx / (float)side, ys = (float)dy / (float)side;
     float cx = (float)x0, cy = (float)y0;
@@ -444,10 +445,12 @@ __device__ void ov_line(const DevFrame& fr, int64_t x0, int64_t y0, int64_t x1, 
         cx += xs;
         cy += ys;
     }
+    return true;
 }
 
 // drawProjection's overlays for one binned triangle (renderer.go:191-216).
-__device__ void draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t slot) {
+// Returns false if a line of the triangle was too long to draw.
+__device__ bool draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t slot) {
     const float* sx = o.sx;
     const float* sy = o.sy;
     int64_t ix[3], iy[3];
@@ -456,10 +459,11 @@ __device__ void draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t 
         iy[j] = go_int(sy[j]);
     }
     const uint32_t code = (slot + 1u) << 1;
+    bool ok = true;
     if (fr.flags & kShowEdges) {
-        ov_line(fr, ix[0], iy[0], ix[1], iy[1], code);
-        ov_line(fr, ix[1], iy[1], ix[2], iy[2], code);
-        ov_line(fr, ix[2], iy[2], ix[0], iy[0], code);
+        ok = ov_line(fr, ix[0], iy[0], ix[1], iy[1], code) & ok;
+        ok = ov_line(fr, ix[1], iy[1], ix[2], iy[2], code) & ok;
+        ok = ov_line(fr, ix[2], iy[2], ix[0], iy[0], code) & ok;
         if (fr.flags & kShowFaces) {
             const float mx = (sx[0] + sx[1] + sx[2]) / 3.0f, my = (sy[0] + sy[1] + sy[2]) / 3.0f;
             ov_rect(fr, wsub(go_int(mx), 1), wsub(go_int(my), 1), code);
@@ -467,6 +471,7 @@ __device__ void draw_overlays(const DevFrame& fr, const OverlayRec& o, uint32_t 
     }
     if (fr.flags & kShowVertices)
         for (int j = 0; j < 3; j++) ov_rect(fr, wsub(ix[j], 1), wsub(iy[j], 1), code | 1u);
+    return ok;
 }
 
 // Tile references of a triangle's tile rectangle, counted for the list scan.
@@ -1137,7 +1142,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
 
 __global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ objs, int nobj,
                                                  const DevFrame* __restrict__ frames,
-                                                 const uint32_t* __restrict__ wg_count) {
+                                                 const uint32_t* __restrict__ wg_count,
+                                                 unsigned long long* __restrict__ stats) {
     const int b = blockIdx.x;
     const int oi = last_le(nobj, b, [&](int i) { return objs[i].face_block; });
     const DevObject& o = objs[oi];
@@ -1147,7 +1153,10 @@ __global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ o
     const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
         const OverlayRec rec = fr.orec[slot0 + i];
-        if (rec.binned) draw_overlays(fr, rec, slot0 + i);
+        // A frame with a line too long is counted on its own, besides the
+        // batch's control word, so a caller can tell which frames failed.
+        if (rec.binned && !draw_overlays(fr, rec, slot0 + i))
+            atomicAdd(&stats[o.frame * kNumStats + kStatLongLines], 1ull);
     }
 }
 
@@ -1897,7 +1906,7 @@ void xcd_tables(const int* n, int nframes, int* table, int* grid) {
 void launch_overlay(const LaunchParams& p, hipStream_t s) {
     if (p.face_blocks == 0) return;
     hipLaunchKernelGGL(k_overlay, dim3(p.face_blocks), dim3(256), 0, s, p.objs, p.nobj, p.frames,
-                       p.wg_count);
+                       p.wg_count, p.stats);
 }
 
 void launch_bin_scan(const LaunchParams& p, hipStream_t s) {

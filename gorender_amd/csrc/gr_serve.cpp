@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -51,6 +52,7 @@ struct Request {
     std::vector<gr_object> objects;
     Target target;
     gr_serve_result result{};
+    std::string error;  // message of a failed request (the worker's, for the waiter)
     bool done = false;
 };
 
@@ -93,7 +95,7 @@ struct gr_server {
     gr_status take_target(size_t bytes, Target* t);
     gr_status launch(std::vector<Request*>& reqs);
     gr_status reap(bool wait);
-    void finish(Request* r, gr_status st);
+    void finish(Request* r, gr_status st, const char* msg = nullptr);
 };
 
 namespace {
@@ -125,8 +127,10 @@ gr_status gr_server::take_target(size_t bytes, Target* t) {
     return GR_OK;
 }
 
-void gr_server::finish(Request* r, gr_status st) {
-    // Caller holds m.
+// Caller holds m. A failure keeps `msg`, else the message of the fail() on
+// this (the worker's) thread that produced `st`.
+void gr_server::finish(Request* r, gr_status st, const char* msg) {
+    if (st) r->error = msg ? msg : gr_last_error();
     r->result.status = (int32_t)st;
     r->result.done_ns = now_ns();
     r->done = true;
@@ -196,7 +200,8 @@ gr_status gr_server::launch(std::vector<Request*>& reqs) {
 }
 
 // Completes the oldest batch if its frames are in host memory (waits for it
-// when `wait`). A batch that overflowed its tile lists is queued again first.
+// when `wait`). A batch that overflowed its tile lists is queued again first
+// (after growing them), whole.
 gr_status gr_server::reap(bool wait) {
     if (inflight.empty()) return GR_OK;
     Batch& b = inflight.front();
@@ -211,14 +216,14 @@ gr_status gr_server::reap(bool wait) {
     unsigned long long needed = 0;
     gr_internal::decode_control(b.h_ctrl, &overflow, &needed, &unsupported);
     std::lock_guard<std::mutex> lk(m);
-    if (overflow && !unsupported) {
+    if (overflow) {
         gr_internal::grow_refs_to(ctx, needed);
         for (auto it = b.reqs.rbegin(); it != b.reqs.rend(); ++it) queue.push_front(*it);
     } else {
         for (size_t k = 0; k < b.reqs.size(); k++) {
             Request* r = b.reqs[k];
-            gr_internal::decode_stats(b.h_stats + k * gr_internal::stats_bytes_per_frame(),
-                                      &r->result.stats);
+            const unsigned char* fs = b.h_stats + k * gr_internal::stats_bytes_per_frame();
+            gr_internal::decode_stats(fs, &r->result.stats);
             used -= r->target.bytes;
             r->result.rgba = (const uint8_t*)r->target.host;
             r->result.depth =
@@ -228,8 +233,12 @@ gr_status gr_server::reap(bool wait) {
             // result until it is released.
             free_dev.emplace(r->target.bytes, r->target.dev);
             r->target.dev = nullptr;
-            finish(r, unsupported ? fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels")
-                                  : GR_OK);
+            // Only the frames with an overlay line too long fail; the others
+            // of the batch are complete.
+            if (gr_internal::frame_long_lines(fs))
+                finish(r, GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+            else
+                finish(r, GR_OK);
         }
         cv_done.notify_all();
     }
@@ -314,10 +323,13 @@ void gr_server::run() {
     (void)hipStreamSynchronize(copy_stream);
     std::lock_guard<std::mutex> lk(m);
     const gr_status st = fatal ? fatal : GR_ERR_INVALID_ARGUMENT;
-    for (Request* r : queue) finish(r, st);
+    const std::string why = fatal ? std::string("server stopped after a device error: ") +
+                                        gr_last_error()
+                                  : std::string("server destroyed before the request ran");
+    for (Request* r : queue) finish(r, st, why.c_str());
     queue.clear();
     for (Batch& b : inflight)
-        for (Request* r : b.reqs) finish(r, st);
+        for (Request* r : b.reqs) finish(r, st, why.c_str());
     cv_done.notify_all();
 }
 
@@ -417,7 +429,8 @@ gr_status gr_server_wait(gr_server* s, int64_t ticket, int32_t timeout_ms, gr_se
     else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
         return fail(GR_ERR_TIMEOUT, "request not finished");
     *out = r->result;
-    return (gr_status)r->result.status;
+    if (r->result.status) return fail((gr_status)r->result.status, r->error);
+    return GR_OK;
 }
 
 gr_status gr_server_release(gr_server* s, int64_t ticket) {

@@ -102,8 +102,9 @@ def test_open_loop_replay(ctx, resident):
 def test_bad_requests_refused_alone(ctx, resident):
     """Requests that a batch would refuse (a reference-tile count other than 1
     or 16; Gouraud lighting over normal indices out of range) are refused at
-    submit, so they cannot fail a shared batch: the server keeps serving the
-    others, each frame equal to the oracle's."""
+    submit, and a request whose overlay lines are too long fails alone inside
+    its batch: the server keeps serving the others, each frame equal to the
+    oracle's."""
     rs, reqs = resident
     v = np.array([[0, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1]], np.float32)
     m = S.MeshData(v, np.array([[0, 1, 2]], np.int32), v.copy(),
@@ -112,6 +113,12 @@ def test_bad_requests_refused_alone(ctx, resident):
     fr = S.FrameData(32, 32)
     lit = PreparedRequest(
         S.SceneData(fr, [m], [], [S.camera_object(fr, 0, (0, 0, 3), (0, 0, -1))]), [h])
+    # Overlay lines between far-off unclipped vertices: this frame fails alone
+    # (GR_ERR_UNSUPPORTED) inside a batch whose other frames complete.
+    import scene_cases as C
+    wide_scene = C.clip_space_soup(flags=C.no(_lib.GR_FRUSTUM_CLIPPING) | _lib.GR_SHOW_EDGES,
+                                   seed=7, w_range=(-1e-4, 1e-4))
+    wide = PreparedRequest(wide_scene, ctx.upload_scene(wide_scene)[0])
     ok = reqs[0]
     tiles = ok.frame.ref_tiles
     with Server(ctx, 1 << 30, 8, 2) as srv:
@@ -125,7 +132,14 @@ def test_bad_requests_refused_alone(ctx, resident):
         with pytest.raises(_lib.GorenderError) as e:
             srv.submit(lit)
         assert e.value.status == _lib.GR_ERR_INDEX_OUT_OF_RANGE
+        tw = srv.submit(wide)
         tickets = [srv.submit(r) for r in reqs]
+        with pytest.raises(_lib.GorenderError) as e:
+            srv.wait_raw(tw)
+        assert e.value.status == _lib.GR_ERR_UNSUPPORTED
+        assert "65536" in str(e.value)
+        srv.release(tw)
         results = [srv.wait(t) for t in tickets]
+    assert max(r.batch_size for r in results) > 1
     for k, res in enumerate(results):
         check_result(k, res)
