@@ -300,12 +300,17 @@ typedef struct {
 gr_status gr_server_create(gr_context *ctx, const gr_server_config *cfg, gr_server **out);
 /* Finishes every submitted request, then frees the server (results included). */
 void gr_server_destroy(gr_server *server);
-/* Queue a request (frame and objects are copied); a frame larger than the
- * budget -> GR_ERR_INVALID_ARGUMENT, a dead mesh handle -> GR_ERR_INVALID_HANDLE. */
+/* Queue a request (frame and objects are copied). It is checked here as
+ * gr_draw_batch would check it, so a bad request is refused alone: a frame
+ * larger than the budget or with bad fields -> GR_ERR_INVALID_ARGUMENT, a dead
+ * mesh or texture handle -> GR_ERR_INVALID_HANDLE, Gouraud lighting over normal
+ * indices out of range -> GR_ERR_INDEX_OUT_OF_RANGE. */
 gr_status gr_server_submit(gr_server *server, const gr_frame *frame, const gr_object *objects,
                            int32_t num_objects, int64_t *ticket);
 /* Wait for a request (timeout_ms < 0: no limit; GR_ERR_TIMEOUT when it passes)
- * and describe its result; returns the request's status. */
+ * and describe its result; returns the request's status, with its message in
+ * gr_last_error. A frame whose overlay lines are too long fails alone
+ * (GR_ERR_UNSUPPORTED); the other frames of its batch complete. */
 gr_status gr_server_wait(gr_server *server, int64_t ticket, int32_t timeout_ms,
                          gr_serve_result *out);
 /* Hand a finished request's host buffers back to the server. */
