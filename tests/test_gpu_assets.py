@@ -55,3 +55,39 @@ def test_obj_upload_and_draw(ctx, tmp_path):
     got = ctx.draw(scene, handles)
     assert_same(scene, got, refcpu.draw(scene))
     assert got[2]["pixels_shaded"] > 1000
+
+
+def test_scene_file_draw(ctx, tmp_path):
+    """A JSON scene manifest (scene.go:56-133): two OBJ meshes, one with a
+    PNG texture and scale, one on the default solid texture, three objects
+    with rotations in degrees; drawn from a camera, bit for bit the oracle's."""
+    import json
+    from gorender_amd.assets import SceneFile
+    ico = S.icosphere(2)
+    p = ico.vertices[:, :3].astype(np.float64)
+    u = (np.arctan2(p[:, 2], p[:, 0]) / (2 * np.pi) + 0.5).astype(np.float32)
+    v = (np.arccos(np.clip(p[:, 1], -1, 1)) / np.pi).astype(np.float32)
+    ico.uvs = np.stack([u, v], -1)[ico.vertex_indices]
+    write_obj(tmp_path / "ico.obj", ico)
+    cube = S.icosphere(1)
+    cube.uvs = np.zeros((cube.num_faces, 3, 2), np.float32)
+    write_obj(tmp_path / "ico1.obj", cube)
+    img = np.random.default_rng(9).integers(0, 256, (8, 16, 4))
+    (tmp_path / "skin.png").write_bytes(pngwrite.encode(img, 6, 8))
+    (tmp_path / "scene.json").write_text(json.dumps({
+        "name": "two-meshes",
+        "meshes": [{"id": "ico", "objFile": "ico.obj", "texture": "skin.png",
+                    "textureScale": 2},
+                   {"id": "plain", "objFile": "ico1.obj"}],
+        "objects": [{"meshID": "ico", "position": [-0.6, 0, 0], "rotation": [10, 35, 0],
+                     "scale": [0.5, 0.5, 0.5]},
+                    {"meshID": "plain", "position": [0.6, 0.1, -0.2], "rotation": [0, 0, 45],
+                     "scale": [0.4, 0.6, 0.4]},
+                    {"meshID": "ico", "position": [0, -0.5, -1], "rotation": [-20, 190, 5],
+                     "scale": [0.7, 0.7, 0.7]}]}))
+    sf = SceneFile(str(tmp_path / "scene.json"))
+    fr = S.FrameData(200, 130)
+    scene = sf.scene(fr, (0, 0.4, 3), (0, -0.13, -1))
+    got = ctx.draw(scene)
+    assert_same(scene, got, refcpu.draw(scene))
+    assert got[2]["pixels_shaded"] > 2000
