@@ -31,6 +31,8 @@ GR_FRUSTUM_CLIPPING = 1 << 4
 GR_SHOW_TEXTURES = 1 << 5
 GR_SHOW_EDGES = 1 << 6
 GR_SHOW_VERTICES = 1 << 7
+GR_SHOW_TILE_BOUNDS = 1 << 8
+GR_SHOW_CROSSHAIR = 1 << 9
 GR_DEFAULT_FLAGS = (GR_SHOW_FACES | GR_BACKFACE_CULLING | GR_LIGHTING | GR_FRUSTUM_CLIPPING
                     | GR_SHOW_TEXTURES)
 
@@ -132,6 +134,8 @@ SIGNATURES = {
     "gr_device_free": (c_int32, [c_void_p, c_void_p]),
     "gr_device_fill": (c_int32, [c_void_p, c_void_p, c_int32, ctypes.c_size_t]),
     "gr_device_copy": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
+    "gr_fog": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_float, c_float,
+                         POINTER(c_uint8)]),
     "gr_obj_open": (c_int32, [ctypes.c_char_p, c_int32, POINTER(c_void_p)]),
     "gr_obj_close": (None, [c_void_p]),
     "gr_obj_num_meshes": (c_int32, [c_void_p]),

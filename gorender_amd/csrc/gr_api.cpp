@@ -377,6 +377,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     dobj.reserve(no);
     int fblocks = 0, tiles = 0;
     size_t slots = 0, okey_words = 0;
+    bool decorate = false;  // a frame asks for tile borders or the crosshair
     for (int f = 0; f < nf; f++) {
         const gr_frame& fr = bf[f].frame;
         if (bf[f].num_objects < 0 || bf[f].first_object < 0 ||
@@ -412,6 +413,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
         d.rgba = (uint8_t*)bf[f].d_rgba;
         d.depth = (float*)bf[f].d_depth;
         // Overlay key buffer offset (bytes + 1; 0 = no overlays).
+        if (fr.flags & (GR_SHOW_TILE_BOUNDS | GR_SHOW_CROSSHAIR)) decorate = true;
         d.okey = nullptr;
         if (fr.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES)) {
             d.okey = (uint32_t*)(uintptr_t)(okey_words * 4 + 1);
@@ -587,6 +589,9 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     if (c->last_pipe >= 0 && c->last_pipe != pi && c->pipes[c->last_pipe].raster_pending)
         GR_HIP(hipStreamWaitEvent(s, c->pipes[c->last_pipe].raster_done, 0));
     if (ok) timed_launch(c, "k_raster", s, [&] { ok = gr::launch_raster(p, s); });
+    // Drawn over the finished frame, before the event the next batch's raster
+    // waits for (it may write the same buffers).
+    if (ok && decorate) timed_launch(c, "k_decorate", s, [&] { gr::launch_decorate(p, s); });
     hipError_t le = hipGetLastError();
     if (!ok || le != hipSuccess) {
         // Leave the tile counters zeroed for the next draw.
@@ -1234,6 +1239,23 @@ gr_status gr_device_copy(gr_context* c, void* dst, const void* src, size_t bytes
     gr_status st = after_draws(c);
     if (st) return st;
     timed_launch(c, "k_copy", c->stream, [&] { gr::launch_copy(dst, src, bytes, c->stream); });
+    GR_HIP(hipGetLastError());
+    return before_draws(c);
+}
+
+gr_status gr_fog(gr_context* c, void* rgba, const void* depth, int32_t width, int32_t height,
+                 float fog_start, float fog_end, const uint8_t color[4]) {
+    if (!c || !rgba || !depth || !color) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    if (width < 1 || height < 1 || width > gr::kMaxDim || height > gr::kMaxDim)
+        return fail(GR_ERR_INVALID_ARGUMENT, "frame size must be within 1..16384");
+    GR_HIP(hipSetDevice(c->device));
+    gr_status st = after_draws(c);
+    if (st) return st;
+    const uint32_t packed = pack_rgba(color);
+    timed_launch(c, "k_fog", c->stream, [&] {
+        gr::launch_fog(rgba, depth, (uint64_t)width * (uint64_t)height, fog_start, fog_end, packed,
+                       c->stream);
+    });
     GR_HIP(hipGetLastError());
     return before_draws(c);
 }

@@ -35,6 +35,8 @@ enum : uint32_t {
     kShowTextures = 1u << 5,
     kShowEdges = 1u << 6,
     kShowVertices = 1u << 7,
+    kShowTileBounds = 1u << 8,
+    kShowCrosshair = 1u << 9,
 };
 
 struct DevMesh {

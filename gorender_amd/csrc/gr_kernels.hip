@@ -1161,6 +1161,91 @@ __global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ o
 }
 
 // ---------------------------------------------------------------------------
+// k_decorate: the end of Draw outside demo mode (renderer.go:476-480)
+// ---------------------------------------------------------------------------
+
+// FrameBuffer.Pixel (rasterizer.go:25-30) straight into the colour buffer:
+// only the flat index is checked, so an x beyond a row wraps into the next.
+__device__ __forceinline__ void fb_pixel(const DevFrame& fr, int64_t x, int64_t y, uint32_t c) {
+    const int64_t idx = wadd(wmul(y, fr.W), x);
+    if (idx > 0 && idx < (int64_t)fr.W * fr.H) ((uint32_t*)fr.rgba)[idx] = c;
+}
+
+// FrameBuffer.Line (rasterizer.go:65-79): the float32 DDA, one lane per line.
+// The lines here stay within the frame's bounds (at most 16385 steps).
+__device__ void fb_line(const DevFrame& fr, int64_t x0, int64_t y0, int64_t x1, int64_t y1,
+                        uint32_t c) {
+    const int64_t dx = wsub(x1, x0), dy = wsub(y1, y0);
+    const int64_t ax = go_abs(dx), ay = go_abs(dy);
+    const int64_t side = ax > ay ? ax : ay;
+    const float xs = (float)dx / (float)side, ys = (float)dy / (float)side;
+    float cx = (float)x0, cy = (float)y0;
+    for (int64_t i = 0; i <= side; i++) {
+        fb_pixel(fr, go_int(cx), go_int(cy), c);
+        cx += xs;
+        cy += ys;
+    }
+}
+
+// One workgroup per frame: the reference tiles' borders (drawTilesBoundaries,
+// renderer.go:422-434), four lanes per tile, in red; then the crosshair
+// (CrossHair, rasterizer.go:209-217) in yellow. Colour only, after the frame.
+__global__ __launch_bounds__(64) void k_decorate(const DevFrame* __restrict__ frames) {
+    const DevFrame& fr = frames[blockIdx.x];
+    if (!(fr.flags & (kShowTileBounds | kShowCrosshair))) return;  // uniform per block
+    const int t = threadIdx.x;
+    if ((fr.flags & kShowTileBounds) && t < 4 * fr.ref_tiles) {
+        constexpr uint32_t kRed = 0xFF0000FFu;  // {255, 0, 0, 255}
+        const float* r = fr.ref_tile_rect[t >> 2];
+        const int64_t sx = go_int(r[0]), sy = go_int(r[1]), ex = go_int(r[2]), ey = go_int(r[3]);
+        switch (t & 3) {
+        case 0: fb_line(fr, sx, sy, ex, sy, kRed); break;
+        case 1: fb_line(fr, ex, sy, ex, ey, kRed); break;
+        case 2: fb_line(fr, ex, ey, sx, ey, kRed); break;
+        default: fb_line(fr, sx, ey, sx, sy, kRed); break;
+        }
+    }
+    __syncthreads();  // the crosshair is drawn over the borders
+    if ((fr.flags & kShowCrosshair) && t < 4) {
+        constexpr uint32_t kYellow = 0xFF00FFFFu;  // {255, 255, 0, 255}
+        constexpr int64_t size = 5, offset = 3;
+        const int64_t x = fr.W / 2, y = fr.H / 2;
+        switch (t) {
+        case 0: fb_line(fr, x - size, y, x - offset, y, kYellow); break;
+        case 1: fb_line(fr, x + offset, y, x + size, y, kYellow); break;
+        case 2: fb_line(fr, x, y - size, x, y - offset, kYellow); break;
+        default: fb_line(fr, x, y + offset, x, y + size, kYellow); break;
+        }
+    }
+}
+
+// FrameBuffer.Fog (rasterizer.go:185-207) with blendRGBA's
+// uint8(float32(a)*(1-f) + float32(b)*f) per channel.
+__global__ __launch_bounds__(256) void k_fog(uint32_t* __restrict__ rgba,
+                                             const float* __restrict__ depth, uint64_t n,
+                                             float fog_start, float fog_end, uint32_t color) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * 256u) {
+        const float d = depth[i];
+        if (d >= fog_start) continue;
+        if (d <= fog_end) {
+            rgba[i] = color;
+            continue;
+        }
+        const float f = 1 - ((fog_end - d) / (fog_end - fog_start));
+        const uint32_t p = rgba[i];
+        uint32_t out = 0;
+        for (int k = 0; k < 4; k++) {
+            const float a = (float)((p >> (8 * k)) & 0xFFu), b = (float)((color >> (8 * k)) & 0xFFu);
+            const float u = a * (1 - f);
+            const float v = b * f;
+            out |= (go_uint8(u + v) & 0xFFu) << (8 * k);
+        }
+        rgba[i] = out;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_bin_scan: tile counts -> list offsets, one workgroup per frame
 // ---------------------------------------------------------------------------
 
@@ -1901,6 +1986,19 @@ void xcd_tables(const int* n, int nframes, int* table, int* grid) {
         longest = std::max(longest, P[nframes]);
     }
     *grid = 8 * longest;
+}
+
+void launch_decorate(const LaunchParams& p, hipStream_t s) {
+    if (p.nframes == 0) return;
+    hipLaunchKernelGGL(k_decorate, dim3(p.nframes), dim3(64), 0, s, p.frames);
+}
+
+void launch_fog(void* rgba, const void* depth, uint64_t n, float fog_start, float fog_end,
+                uint32_t color, hipStream_t s) {
+    if (n == 0) return;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fog, dim3((unsigned)blocks), dim3(256), 0, s, (uint32_t*)rgba,
+                       (const float*)depth, n, fog_start, fog_end, color);
 }
 
 void launch_overlay(const LaunchParams& p, hipStream_t s) {

@@ -52,6 +52,9 @@ void launch_check_div(uint64_t n, uint64_t seed, unsigned long long* mismatches,
                       hipStream_t s);
 void launch_bin_scan(const LaunchParams& p, hipStream_t s);
 void launch_overlay(const LaunchParams& p, hipStream_t s);
+void launch_decorate(const LaunchParams& p, hipStream_t s);
+void launch_fog(void* rgba, const void* depth, uint64_t n, float fog_start, float fog_end,
+                uint32_t color, hipStream_t s);
 void launch_bin_fill(const LaunchParams& p, hipStream_t s);
 bool launch_raster_parts(const LaunchParams& p, hipStream_t s);
 bool launch_raster(const LaunchParams& p, hipStream_t s);

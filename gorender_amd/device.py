@@ -207,6 +207,14 @@ class Context:
         """Device colour (H*W*4 bytes) and depth (H*W floats) buffers."""
         return self.alloc(width * height * 4), self.alloc(width * height * 4)
 
+    def fog(self, rgba: "DeviceBuffer", depth: "DeviceBuffer", width: int, height: int,
+            fog_start: float, fog_end: float, color) -> None:
+        """FrameBuffer.Fog (rasterizer.go:185-207) over a frame in device
+        memory, on the context stream."""
+        c = (ctypes.c_uint8 * 4)(*[int(v) for v in color])
+        _lib.check(self.lib.gr_fog(self.handle, rgba.ptr, depth.ptr, width, height,
+                                   float(fog_start), float(fog_end), c))
+
     # ---- measurement -------------------------------------------------------
     def set_kernel_timing(self, on: bool) -> None:
         _lib.check(self.lib.gr_set_kernel_timing(self.handle, 1 if on else 0))

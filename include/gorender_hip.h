@@ -72,6 +72,12 @@ enum {
     GR_SHOW_VERTICES = 1u << 7, /* debug overlay: vertex dots (renderer.go:212-216); an
                                    overlay line longer than 65536 pixels (unclipped
                                    geometry far off screen) -> GR_ERR_UNSUPPORTED */
+    /* The end of Draw outside demo mode (renderer.go:476-480), drawn after the
+     * frame, colour only: the reference tiles' borders in red
+     * (drawTilesBoundaries, renderer.go:422-434), then the yellow crosshair
+     * (FrameBuffer.CrossHair, rasterizer.go:209-217). */
+    GR_SHOW_TILE_BOUNDS = 1u << 8,
+    GR_SHOW_CROSSHAIR = 1u << 9,
 };
 #define GR_DEFAULT_FLAGS \
     (GR_SHOW_FACES | GR_BACKFACE_CULLING | GR_LIGHTING | GR_FRUSTUM_CLIPPING | GR_SHOW_TEXTURES)
@@ -235,6 +241,11 @@ gr_status gr_device_fill(gr_context *ctx, void *ptr, int32_t value, size_t bytes
 /* Device -> device copy of `bytes` bytes on the context stream (asynchronous);
  * the regions must not overlap. */
 gr_status gr_device_copy(gr_context *ctx, void *dst, const void *src, size_t bytes);
+/* FrameBuffer.Fog (rasterizer.go:185-207) over a drawn frame in device
+ * memory, on the context stream (asynchronous): depth >= fog_start keeps the
+ * pixel, depth <= fog_end takes `color`, depths between blend toward it. */
+gr_status gr_fog(gr_context *ctx, void *rgba, const void *depth, int32_t width, int32_t height,
+                 float fog_start, float fog_end, const uint8_t color[4]);
 /* Wait for the context stream, then copy device -> host. */
 gr_status gr_copy_to_host(gr_context *ctx, void *dst, const void *src, size_t bytes);
 

@@ -731,6 +731,58 @@ void rc_tile_bounds(int32_t tile, int32_t num_tiles, int32_t width, int32_t heig
     if (end->y > (float)height) end->y = (float)height;
 }
 
+/* renderer.go:422-434 (drawTilesBoundaries) and rasterizer.go:209-217
+ * (CrossHair), in the order of renderer.go:476-480. int(float32) of the tile
+ * bounds, W/2 and H/2 as Go's integer division. */
+void rc_frame_decorations(const rc_frame *fr, uint8_t *rgba) {
+    const int32_t W = fr->width, H = fr->height;
+    if (fr->flags & RC_SHOW_TILE_BOUNDS) {
+        const uint8_t red[4] = {255, 0, 0, 255};
+        for (int32_t i = 0; i < fr->ref_tiles; i++) {
+            rc_vec2 s, e;
+            rc_tile_bounds(i, fr->ref_tiles, W, H, &s, &e);
+            int64_t sx = rc_go_int(s.x), sy = rc_go_int(s.y);
+            int64_t ex = rc_go_int(e.x), ey = rc_go_int(e.y);
+            rc_line(W, H, rgba, sx, sy, ex, sy, red);
+            rc_line(W, H, rgba, ex, sy, ex, ey, red);
+            rc_line(W, H, rgba, ex, ey, sx, ey, red);
+            rc_line(W, H, rgba, sx, ey, sx, sy, red);
+        }
+    }
+    if (fr->flags & RC_SHOW_CROSSHAIR) {
+        const uint8_t yellow[4] = {255, 255, 0, 255};
+        const int64_t size = 5, offset = 3, x = W / 2, y = H / 2;
+        rc_line(W, H, rgba, x - size, y, x - offset, y, yellow);
+        rc_line(W, H, rgba, x + offset, y, x + size, y, yellow);
+        rc_line(W, H, rgba, x, y - size, x, y - offset, yellow);
+        rc_line(W, H, rgba, x, y + offset, x, y + size, yellow);
+    }
+}
+
+/* rasterizer.go:185-191: uint8(float32(a)*(1-f) + float32(b)*f) per channel. */
+static uint8_t blend8(uint8_t a, uint8_t b, float f) {
+    float t = (float)a * (1 - f);
+    float u = (float)b * f;
+    return rc_go_uint8(t + u);
+}
+
+/* rasterizer.go:193-207 */
+void rc_fog(int32_t width, int32_t height, uint8_t *rgba, const float *depth, float fog_start,
+            float fog_end, const uint8_t c[4]) {
+    for (int64_t i = 0; i < (int64_t)width * height; i++) {
+        float d = depth[i];
+        uint8_t *p = rgba + 4 * i;
+        if (d >= fog_start) {
+            /* noop */
+        } else if (d <= fog_end) {
+            memcpy(p, c, 4);
+        } else {
+            float f = 1 - ((fog_end - d) / (fog_end - fog_start));
+            for (int k = 0; k < 4; k++) p[k] = blend8(p[k], c[k], f);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* renderer.go: geometry stage and the frame                                  */
 /* ------------------------------------------------------------------------- */
@@ -1029,6 +1081,7 @@ int rc_draw(const rc_frame *fr, const rc_mesh *meshes, int32_t nm, const rc_text
         free(written);
     }
     free(list.v);
+    rc_frame_decorations(fr, rgba);
     if (st) *st = s;
     return 0;
 }
@@ -1149,6 +1202,7 @@ int rc_draw_parallel(const rc_frame *fr, const rc_mesh *meshes, int32_t nm,
     }
     free(jobs);
     if (rc) return rc;
+    rc_frame_decorations(fr, rgba);
     if (st) *st = s;
     return 0;
 }

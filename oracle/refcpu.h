@@ -72,6 +72,8 @@ enum {
     RC_SHOW_TEXTURES = 1u << 5,
     RC_SHOW_EDGES = 1u << 6,
     RC_SHOW_VERTICES = 1u << 7,
+    RC_SHOW_TILE_BOUNDS = 1u << 8, /* drawTilesBoundaries (renderer.go:422-434) */
+    RC_SHOW_CROSSHAIR = 1u << 9,   /* FrameBuffer.CrossHair (rasterizer.go:209-217) */
 };
 
 typedef struct {
@@ -163,6 +165,13 @@ void rc_line(int32_t width, int32_t height, uint8_t *rgba, int64_t x0, int64_t y
              int64_t y1, const uint8_t c[4]);
 void rc_tile_bounds(int32_t tile, int32_t num_tiles, int32_t width, int32_t height, rc_vec2 *start,
                     rc_vec2 *end);
+
+/* The end of Draw outside demo mode (renderer.go:476-480): the reference
+ * tiles' borders in red, then the crosshair in yellow, colour only. */
+void rc_frame_decorations(const rc_frame *frame, uint8_t *rgba);
+/* FrameBuffer.Fog (rasterizer.go:185-207), over colour and depth. */
+void rc_fog(int32_t width, int32_t height, uint8_t *rgba, const float *depth, float fog_start,
+            float fog_end, const uint8_t c[4]);
 
 /* ---- whole frame (renderer.go:443-483, sequential path) ---- */
 int rc_draw(const rc_frame *frame, const rc_mesh *meshes, int32_t num_meshes,

@@ -123,6 +123,7 @@ def load() -> ctypes.CDLL:
         "rc_line": (None, [c_int32, c_int32, P, c_int64, c_int64, c_int64, c_int64, P]),
         "rc_tile_bounds": (None, [c_int32, c_int32, c_int32, c_int32, POINTER(rc_vec2),
                                   POINTER(rc_vec2)]),
+        "rc_fog": (None, [c_int32, c_int32, P, P, ctypes.c_float, ctypes.c_float, P]),
         "rc_draw": (c_int32, [POINTER(rc_frame), P, c_int32, P, c_int32, P, c_int32, P, P,
                               POINTER(rc_stats)]),
         "rc_draw_parallel": (c_int32, [POINTER(rc_frame), P, c_int32, P, c_int32, P, c_int32,
@@ -253,3 +254,15 @@ def draw(scene, prepared: PreparedScene = None, parallel: int = 0, tiles=(4, 4))
     if rc != 0:
         raise RuntimeError(f"oracle rc_draw failed with code {rc}")
     return rgba, depth, st.as_dict()
+
+
+def fog(rgba: np.ndarray, depth: np.ndarray, fog_start: float, fog_end: float, color) -> np.ndarray:
+    """FrameBuffer.Fog (rasterizer.go:185-207) on a copy of an (H, W, 4) frame."""
+    lib = load()
+    out = np.ascontiguousarray(rgba, np.uint8).copy()
+    d = np.ascontiguousarray(depth, np.float32)
+    c = np.asarray(color, np.uint8)
+    h, w = out.shape[:2]
+    lib.rc_fog(w, h, _p(out), _p(d), float(np.float32(fog_start)), float(np.float32(fog_end)),
+               _p(c))
+    return out

@@ -143,3 +143,65 @@ def test_errors(ctx):
     with pytest.raises(_lib.GorenderError) as e:
         ctx.draw(scene, [h + 1000])
     assert e.value.status == _lib.GR_ERR_INVALID_HANDLE
+
+
+@pytest.mark.parametrize("flags,ref_tiles,size", [
+    (_lib.GR_SHOW_CROSSHAIR, 16, (160, 96)),
+    (_lib.GR_SHOW_TILE_BOUNDS, 16, (77, 53)),
+    (_lib.GR_SHOW_TILE_BOUNDS | _lib.GR_SHOW_CROSSHAIR, 1, (64, 48)),
+    (_lib.GR_SHOW_TILE_BOUNDS | _lib.GR_SHOW_CROSSHAIR | _lib.GR_SHOW_EDGES, 16, (120, 90)),
+    (_lib.GR_SHOW_TILE_BOUNDS | _lib.GR_SHOW_CROSSHAIR, 16, (5, 7)),
+])
+def test_decorations(ctx, flags, ref_tiles, size):
+    """Tile borders and crosshair (renderer.go:422-434,476-480;
+    rasterizer.go:209-217) over the drawn frame, as the oracle draws them."""
+    import scene_cases as C
+    scene = C.sphere(w=size[0], h=size[1], eye=(0, 0, 2.4), flags=C.ALL | flags)
+    scene.frame.ref_tiles = ref_tiles
+    assert_same(scene, ctx.draw(scene), refcpu.draw(scene))
+
+
+def test_decorations_batched(ctx):
+    """Decorated and plain frames in one batch into shared buffers: each frame
+    keeps only its own decorations, and a following draw into the same
+    buffers is not painted over."""
+    import scene_cases as C
+    frames = [C.sphere(w=96, h=64, seed=s, flags=C.ALL | f) for s, f in
+              ((21, _lib.GR_SHOW_CROSSHAIR), (22, 0),
+               (23, _lib.GR_SHOW_TILE_BOUNDS | _lib.GR_SHOW_CROSSHAIR))]
+    handles = [ctx.upload_scene(s)[0] for s in frames]
+    outs = [ctx.frame_buffers(96, 64) for _ in frames]
+    ctx.draw_batch(frames, handles, [(o[0].ptr, o[1].ptr) for o in outs])
+    # The undecorated frame again into the first frame's buffers, right after.
+    ctx.draw_batch(frames[1:2], handles[1:2], [(outs[0][0].ptr, outs[0][1].ptr)])
+    ctx.synchronize()
+    for k, src in ((0, 1), (1, 1), (2, 2)):
+        want = refcpu.draw(frames[src])
+        got = outs[k][0].to_numpy(np.uint8, (64, 96, 4))
+        assert np.array_equal(got, want[0]), f"buffers {k} differ from frame {src}"
+    for rg, dp in outs:
+        rg.free()
+        dp.free()
+
+
+def test_fog(ctx):
+    """gr_fog (FrameBuffer.Fog, rasterizer.go:185-207) over a GPU frame equals
+    the oracle's fog over the oracle's frame; the depths span keep, blend and
+    fog colour (the background's -1 included)."""
+    import scene_cases as C
+    scene = C.sphere(w=120, h=90, eye=(0, 0, 2.4))
+    want_rgba, want_depth, _ = refcpu.draw(scene)
+    d = want_depth[want_depth > 0]
+    start, end = float(np.quantile(d, 0.7)), float(np.quantile(d, 0.2))
+    want = refcpu.fog(want_rgba, want_depth, start, end, (100, 120, 140, 255))
+    h, w = 90, 120
+    handles = ctx.upload_scene(scene)[0]
+    rg, dp = ctx.frame_buffers(w, h)
+    ctx.draw_device(scene, handles, rg.ptr, dp.ptr)
+    ctx.fog(rg, dp, w, h, start, end, (100, 120, 140, 255))
+    ctx.synchronize()
+    got = rg.to_numpy(np.uint8, (h, w, 4))
+    assert np.array_equal(got, want)
+    assert (got != want_rgba).any(axis=-1).sum() > 1000
+    rg.free()
+    dp.free()

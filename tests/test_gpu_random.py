@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 
 FLAG_BITS = [_lib.GR_SHOW_FACES, _lib.GR_BACKFACE_CULLING, _lib.GR_LIGHTING,
              _lib.GR_FLAT_SHADING, _lib.GR_FRUSTUM_CLIPPING, _lib.GR_SHOW_TEXTURES,
-             _lib.GR_SHOW_EDGES, _lib.GR_SHOW_VERTICES]
-FLAG_P = [0.85, 0.7, 0.7, 0.3, 0.85, 0.8, 0.15, 0.15]
+             _lib.GR_SHOW_EDGES, _lib.GR_SHOW_VERTICES, _lib.GR_SHOW_TILE_BOUNDS,
+             _lib.GR_SHOW_CROSSHAIR]
+FLAG_P = [0.85, 0.7, 0.7, 0.3, 0.85, 0.8, 0.15, 0.15, 0.15, 0.15]
 TILES = [(8, 8), (16, 16), (32, 16), (16, 32), (32, 32), (64, 16), (64, 64), (8, 64)]
 N_SINGLE = 32
 N_BATCH = 8
