@@ -204,7 +204,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this render path runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md)"}))
+                                   "(sharding frames over several GPUs is a design note, "
+                                   "not built)"}))
         sys.exit(2)
 
     import numpy as np
