@@ -73,6 +73,28 @@ def measured_traffic(kernel, cfg, frames_per_launch):
     return None
 
 
+# Wave64 VALU issue of the whole chip: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# VALU instruction per SIMD every 4 cycles.
+VALU_PEAK_GINSTR_S = 256 * 4 * 2.4 / 4
+
+
+def measured_valu(kernel, cfg, frames_per_launch):
+    """VALU wave-instructions per launch of `kernel` (SQ_INSTS_VALU) from the
+    committed rocprofv3 counter summary of the headline batch, when it was
+    taken with this tile and 64 frames per launch; else None."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_config2.txt")
+    if frames_per_launch != 64 or not os.path.exists(path):
+        return None
+    name = f"{kernel}<{cfg.tile_w}, {cfg.tile_h}>" if kernel.startswith("k_raster") else kernel
+    tile_ok = False
+    for line in open(path):
+        if line.startswith("# raster tile "):
+            tile_ok = line.split()[-1] == f"{cfg.tile_w}x{cfg.tile_h}"
+        elif tile_ok and line.startswith(name + " ") and "SQ_INSTS_VALU=" in line:
+            return int(line.split("SQ_INSTS_VALU=")[1].split()[0])
+    return None
+
+
 def other_configs(ctx, scenes):
     """Throughput of configs 3-5 on the same context: config 3 (64 instances,
     ~780k triangles, 1920x1080) drawn one frame per call; config 4 (16 frames of
@@ -393,6 +415,15 @@ def main():
     dom_bytes = alg_bytes.get(dom_name, 0) * frames_per_launch
     achieved = dom_bytes / avg_launch_s / 1e9 if dom_bytes else 0.0
 
+    n_valu = measured_valu(dom_name, ctx.config(), frames_per_launch)
+    valu_issue = None
+    if n_valu:
+        ach = n_valu / avg_launch_s / 1e9
+        valu_issue = {"kernel": dom_name, "instr_per_launch": n_valu,
+                      "achieved_ginstr_s": round(ach, 1), "peak_ginstr_s": VALU_PEAK_GINSTR_S,
+                      "frac": round(ach / VALU_PEAK_GINSTR_S, 4),
+                      "source": "SQ_INSTS_VALU, profiles/sq_counters_config2.txt"}
+
     cpu, cpu_forms = None, None
     if not args.no_cpu and not args.profile:
         from oracle import refcpu
@@ -467,6 +498,9 @@ def main():
                      "traffic": measured_traffic(dom_name, ctx.config(), frames_per_launch),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "algorithmic_bytes_per_launch": dom_bytes},
+        # The same kernel against its compute ceiling: VALU instructions per
+        # launch (counter profile) over the live launch time.
+        "valu_issue": valu_issue,
         "cpu_baseline": cpu,
         "cpu_baseline_forms": cpu_forms,
         "tile_sweep_fps": tile_sweep or None,
