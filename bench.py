@@ -152,7 +152,8 @@ def serving(ctx, requests, mh):
     objects, 640x360 to 1920x1080), latency from submission to colour and depth
     in host memory. 'serial' draws one request at a time (gr_draw: launch, wait,
     copy out); 'saturated' submits a burst of 512 at once; the qps lines are
-    open-loop Poisson streams of the stated rate (gr_server_replay)."""
+    open-loop Poisson streams of the stated rate (gr_server_replay);
+    'saturated_colour_only' is the burst with colour read back alone."""
     import numpy as np
     from gorender_amd.serve import PreparedRequest, Server, poisson_arrivals, replay
 
@@ -183,6 +184,13 @@ def serving(ctx, requests, mh):
             t = time.perf_counter()
             lat, bs = replay(srv, reqs, poisson_arrivals(n, qps, seed=qps))
             res[f"qps_{qps}"] = summary(lat, time.perf_counter() - t, n, bs)
+    # A window's present needs colour only: the same burst without depth readback.
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2,
+                colour_only=True) as srv:
+        replay(srv, reqs, np.zeros(64))
+        t = time.perf_counter()
+        lat, bs = replay(srv, reqs, np.zeros(512))
+        res["saturated_colour_only"] = summary(lat, time.perf_counter() - t, 512, bs)
     res["config"] = {"target_budget_bytes": 2 << 30, "max_batch": 16, "max_inflight": 2,
                      "distinct_requests": len(requests)}
     return res

@@ -88,7 +88,8 @@ class gr_config(Structure):
 
 
 class gr_server_config(Structure):
-    _fields_ = [("target_budget", c_int64), ("max_batch", c_int32), ("max_inflight", c_int32)]
+    _fields_ = [("target_budget", c_int64), ("max_batch", c_int32), ("max_inflight", c_int32),
+                ("colour_only", c_int32), ("reserved", c_int32)]
 
 
 class gr_serve_result(Structure):

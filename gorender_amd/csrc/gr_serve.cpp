@@ -184,11 +184,15 @@ gr_status gr_server::launch(std::vector<Request*>& reqs) {
         hipEventRecord(b.rendered, s) != hipSuccess ||
         hipStreamWaitEvent(copy_stream, b.rendered, 0) != hipSuccess)
         return fail(GR_ERR_DEVICE, "server batch readback");
-    // Frames to pinned host memory on the copy stream, overlapping the next batch.
-    for (Request* r : reqs)
-        if (hipMemcpyAsync(r->target.host, r->target.dev, r->target.bytes, hipMemcpyDeviceToHost,
+    // Frames to pinned host memory on the copy stream, overlapping the next
+    // batch: colour, then depth unless the server reads back colour only.
+    for (Request* r : reqs) {
+        const size_t n = cfg.colour_only ? (size_t)r->frame.width * r->frame.height * 4
+                                         : r->target.bytes;
+        if (hipMemcpyAsync(r->target.host, r->target.dev, n, hipMemcpyDeviceToHost,
                            copy_stream) != hipSuccess)
             return fail(GR_ERR_DEVICE, "server frame readback");
+    }
     if (hipEventRecord(b.copied, copy_stream) != hipSuccess)
         return fail(GR_ERR_DEVICE, "server batch event");
     for (Request* r : reqs) {
@@ -227,8 +231,9 @@ gr_status gr_server::reap(bool wait) {
             used -= r->target.bytes;
             r->result.rgba = (const uint8_t*)r->target.host;
             r->result.depth =
-                (const float*)((const char*)r->target.host +
-                               (size_t)r->frame.width * r->frame.height * 4);
+                cfg.colour_only ? nullptr
+                                : (const float*)((const char*)r->target.host +
+                                                 (size_t)r->frame.width * r->frame.height * 4);
             // The device half returns to the pool; the host half stays with the
             // result until it is released.
             free_dev.emplace(r->target.bytes, r->target.dev);
@@ -357,6 +362,8 @@ gr_status gr_server_create(gr_context* ctx, const gr_server_config* cfg, gr_serv
     *out = nullptr;
     if (cfg->target_budget < 8 || cfg->max_batch < 1 || cfg->max_inflight < 1)
         return fail(GR_ERR_INVALID_ARGUMENT, "server budget, max_batch and max_inflight must be positive");
+    if ((cfg->colour_only != 0 && cfg->colour_only != 1) || cfg->reserved != 0)
+        return fail(GR_ERR_INVALID_ARGUMENT, "colour_only must be 0 or 1 and reserved 0");
     try {
         std::unique_ptr<gr_server> s(new gr_server);
         s->ctx = ctx;

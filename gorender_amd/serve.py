@@ -21,7 +21,7 @@ from .scenes import SceneData
 @dataclass
 class ServeResult:
     rgba: np.ndarray       # (H, W, 4) uint8, copied out of the server's pinned buffer
-    depth: np.ndarray      # (H, W) float32
+    depth: Optional[np.ndarray]  # (H, W) float32; None when the server reads colour only
     stats: dict
     submit_ns: int
     start_ns: int
@@ -45,10 +45,10 @@ class PreparedRequest:
 
 class Server:
     def __init__(self, ctx: Context, target_budget: int = 1 << 30, max_batch: int = 16,
-                 max_inflight: int = 2):
+                 max_inflight: int = 2, colour_only: bool = False):
         self.ctx = ctx
         self.lib = ctx.lib
-        cfg = _lib.gr_server_config(target_budget, max_batch, max_inflight)
+        cfg = _lib.gr_server_config(target_budget, max_batch, max_inflight, int(colour_only), 0)
         h = ctypes.c_void_p()
         _lib.check(self.lib.gr_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
@@ -92,8 +92,9 @@ class Server:
         if copy:
             rgba = np.ctypeslib.as_array(ctypes.cast(r.rgba, ctypes.POINTER(ctypes.c_uint8)),
                                          shape=(h, w, 4)).copy()
-            depth = np.ctypeslib.as_array(ctypes.cast(r.depth, ctypes.POINTER(ctypes.c_float)),
-                                          shape=(h, w)).copy()
+            if r.depth:
+                depth = np.ctypeslib.as_array(
+                    ctypes.cast(r.depth, ctypes.POINTER(ctypes.c_float)), shape=(h, w)).copy()
         out = ServeResult(rgba, depth, r.stats.as_dict(), r.submit_ns, r.start_ns, r.done_ns,
                           r.batch_size)
         self.release(ticket)

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GR_ABI_VERSION 2
+#define GR_ABI_VERSION 3
 
 typedef enum {
     GR_OK = 0,
@@ -296,6 +296,9 @@ typedef struct {
     int64_t target_budget; /* bytes of device colour + depth targets admitted at once */
     int32_t max_batch;     /* requests per batched launch */
     int32_t max_inflight;  /* launches queued on the device at once */
+    int32_t colour_only;   /* 1: read back colour only (result depth = NULL), as a
+                              window's present needs (main.go:296-298); 0: both */
+    int32_t reserved;      /* 0 */
 } gr_server_config;
 typedef struct {
     int32_t status;        /* gr_status of the request */

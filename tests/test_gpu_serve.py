@@ -2,6 +2,7 @@
 oracle's for that request drawn alone (committed digests), whatever batch it
 shares, under tight target budgets, and after a tile-list overflow makes the
 server grow its capacity and draw a batch again."""
+import ctypes
 import json
 import os
 
@@ -143,3 +144,26 @@ def test_bad_requests_refused_alone(ctx, resident):
     assert max(r.batch_size for r in results) > 1
     for k, res in enumerate(results):
         check_result(k, res)
+
+
+def test_colour_only(ctx, resident):
+    """colour_only: the frames come back without depth (result depth NULL), the
+    colour and counters of each still equal to the oracle's."""
+    import hashlib
+    rs, reqs = resident
+    with Server(ctx, 1 << 30, 8, 2, colour_only=True) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    assert max(r.batch_size for r in results) > 1
+    for k, res in enumerate(results):
+        assert res.depth is None
+        want = GOLDEN[f"config5-{k}"]
+        assert hashlib.sha256(np.ascontiguousarray(res.rgba).tobytes()).hexdigest() == want["rgba"]
+        assert res.stats["tpf"] == want["tpf"] and res.stats["triangles"] == want["triangles"]
+
+
+def test_server_config_checked(ctx):
+    bad = _lib.gr_server_config(1 << 20, 4, 2, 2, 0)
+    h = ctypes.c_void_p()
+    assert ctx.lib.gr_server_create(ctx.handle, ctypes.byref(bad), ctypes.byref(h)) == \
+        _lib.GR_ERR_INVALID_ARGUMENT
