@@ -1545,14 +1545,14 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
         const uint32_t k = (head + (uint32_t)lane) & 127u;
         const uint32_t pk = q[0][k];
         const int32_t f12 = (int32_t)q[1][k], f20 = (int32_t)q[2][k];
-        const uint32_t i = pk >> 12, px = (pk >> 6) & 63u, py = pk & 63u;
+        const uint32_t i = pk >> 12, px_idx = pk & 0xFFFu;
         const uint4 d = lrec[i * 5], rc = lrec[i * 5 + 1];
         const Bary b = bary((float)f12, (float)f20,
                             make_float4(__uint_as_float(d.x), __uint_as_float(d.y),
                                         __uint_as_float(d.z), __uint_as_float(d.w)),
                             make_float4(__uint_as_float(rc.x), __uint_as_float(rc.y),
                                         __uint_as_float(rc.z), __uint_as_float(rc.w)));
-        if (b.zrec >= -1.0f) atomicMax(&key[py * TW + px], depth_key(b.zrec, lslot[i] + 1u));
+        if (b.zrec >= -1.0f) atomicMax(&key[px_idx], depth_key(b.zrec, lslot[i] + 1u));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1581,6 +1581,8 @@ __device__ __forceinline__ unsigned long long raster_list(
     unsigned long long* key, uint4* lrec, uint32_t* lslot, uint32_t (*q)[128], int lane,
     int wave) {
     constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
+    static_assert(TW * TH <= 4096 && CHUNK <= (1 << 20),
+                  "fragment codes hold the pixel index in 12 bits");
     unsigned long long covered_lane = 0;  // wide path, per lane
     uint32_t covered_wave = 0;            // int32 path, wave-uniform
     for (uint32_t base = 0; base < cnt; base += CHUNK) {
@@ -1654,7 +1656,10 @@ __device__ __forceinline__ unsigned long long raster_list(
             int f2 = __mul24(A2, px) + __mul24(B2, py) + C2;
             const int sx0 = A0 * lpr, sx1 = A1 * lpr, sx2 = A2 * lpr;
             const int sy0 = B0 * rows, sy1 = B1 * rows, sy2 = B2 * rows;
-            uint32_t prow = (i << 12) | ((uint32_t)(px - X0) << 6) | (uint32_t)(py - Y0);
+            // Fragment code: list index << 12 | the pixel's index in the tile
+            // (y * TW + x, below 4096 for every tile size), exact in every lane
+            // that enqueues (its pixel lies inside the tile).
+            uint32_t prow = (i << 12) + (uint32_t)(py - Y0) * (uint32_t)TW + (uint32_t)(px - X0);
             for (int yb = y0; yb <= y1; yb += rows) {
                 // Negative exactly when the lane's row is inside the box.
                 const int gy = ly - (y1 - yb) - 1;
@@ -1688,12 +1693,12 @@ __device__ __forceinline__ unsigned long long raster_list(
                     g0 += sx0;
                     g1 += sx1;
                     g2 += sx2;
-                    pk += (uint32_t)lpr << 6;
+                    pk += (uint32_t)lpr;
                 }
                 f0 += sy0;
                 f1 += sy1;
                 f2 += sy2;
-                prow += (uint32_t)rows;
+                prow += (uint32_t)rows * (uint32_t)TW;
             }
         }
         while (qtail != qhead) {
