@@ -820,6 +820,41 @@ static int tri_overlaps(const rc_vec4 p[3], rc_vec2 start, rc_vec2 end) {
     return maxx >= start.x && minx <= end.x && maxy >= start.y && miny <= end.y;
 }
 
+/* renderer.go:326-335 (one corner: ambientStrength + vn.DotProduct(light) *
+ * diffuseStrength) and renderer.go:337-341 (flat: the same two operations on
+ * the face normal): the world-space normal is normalised in 4-D with its W
+ * (vector.go:115-125), then its X, Y, Z are dotted with the light. */
+float rc_light_intensity(rc_vec4 world_normal, rc_vec3 light, float ambient, float diffuse) {
+    rc_vec3 n = v4_xyz(rc_vec4_normalize(world_normal));
+    float d = v3_dot(n, light) * diffuse;
+    return ambient + d;
+}
+
+/* renderer.go:361-370: all four components divided by W (vector.go:103-105),
+ * the screen matrix applied with all 16 products (matrix.go:167-173), then W
+ * restored to the clip-space W. */
+rc_vec4 rc_project_point(rc_vec4 clip, const rc_matrix *screen) {
+    float orig_w = clip.w;
+    rc_vec4 p = v4_div(clip, clip.w);
+    rc_transform_inplace(screen, &p);
+    p.w = orig_w;
+    return p;
+}
+
+/* renderer.go:226-244 over the num_tiles reference tiles of
+ * calculateTileBoundaries (renderer.go:50-76): the number of tiles whose
+ * inclusive float rectangle the triangle's float bounding box overlaps, i.e.
+ * what one triangle adds to TPF (renderer.go:436-441). */
+int64_t rc_tile_refs(const rc_vec4 points[3], int32_t num_tiles, int32_t width, int32_t height) {
+    int64_t n = 0;
+    for (int32_t t = 0; t < num_tiles; t++) {
+        rc_vec2 s, e;
+        rc_tile_bounds(t, num_tiles, width, height, &s, &e);
+        if (tri_overlaps(points, s, e)) n++;
+    }
+    return n;
+}
+
 typedef struct {
     const rc_frame *frame;
     const rc_mesh *mesh;
@@ -888,10 +923,7 @@ static void project_object(rc_project_job *job) {
     rc_transform_batch(&obj->world, wvn, N);
     int has_vn = N != 0;
 
-    rc_vec2 tstart[16], tend[16];
     int32_t nt = fr->ref_tiles == 16 ? 16 : 1;
-    for (int32_t t = 0; t < nt; t++)
-        rc_tile_bounds(t, nt, fr->width, fr->height, &tstart[t], &tend[t]);
     rc_vec2 full_s, full_e;
     rc_tile_bounds(0, 1, fr->width, fr->height, &full_s, &full_e);
 
@@ -913,13 +945,10 @@ static void project_object(rc_project_job *job) {
                         job->error = 2; /* reference: index out of range panic */
                         goto done;
                     }
-                    rc_vec3 n = v4_xyz(rc_vec4_normalize(wvn[ni[k]]));
-                    vint[k] = fr->ambient + v3_dot(n, fr->light) * fr->diffuse;
+                    vint[k] = rc_light_intensity(wvn[ni[k]], fr->light, fr->ambient, fr->diffuse);
                 }
             } else {
-                rc_vec3 fn = v4_xyz(rc_vec4_normalize(wfn[fi]));
-                float diffuse = v3_dot(fn, fr->light) * fr->diffuse;
-                float intensity = fr->ambient + diffuse;
+                float intensity = rc_light_intensity(wfn[fi], fr->light, fr->ambient, fr->diffuse);
                 vint[0] = vint[1] = vint[2] = intensity;
             }
         } else {
@@ -946,18 +975,12 @@ static void project_object(rc_project_job *job) {
         for (int i = 0; i < count; i++) {
             rc_tri t;
             for (int j = 0; j < 3; j++) {
-                rc_vec4 p = cv[i][j];
-                float orig_w = p.w;
-                p = v4_div(p, p.w);
-                rc_transform_inplace(&fr->screen, &p);
-                p.w = orig_w;
-                t.points[j] = p;
+                t.points[j] = rc_project_point(cv[i][j], &fr->screen);
                 t.uvs[j] = cuv[i][j];
                 t.intensity[j] = cint[i][j];
             }
             t.texture = mesh->face_textures ? mesh->face_textures[fi] : -1;
-            for (int32_t k = 0; k < nt; k++)
-                if (tri_overlaps(t.points, tstart[k], tend[k])) job->tpf++;
+            job->tpf += rc_tile_refs(t.points, nt, fr->width, fr->height);
             if (tri_overlaps(t.points, full_s, full_e)) {
                 job->binned++;
                 if (job->bins) {

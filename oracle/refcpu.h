@@ -173,6 +173,14 @@ void rc_frame_decorations(const rc_frame *frame, uint8_t *rgba);
 void rc_fog(int32_t width, int32_t height, uint8_t *rgba, const float *depth, float fog_start,
             float fog_end, const uint8_t c[4]);
 
+/* ---- geometry stage pieces (renderer.go:226-244,312-370) ---- */
+/* Lighting of one corner (Gouraud) or face (flat) from its world-space normal. */
+float rc_light_intensity(rc_vec4 world_normal, rc_vec3 light, float ambient, float diffuse);
+/* Perspective divide + screen transform of a clip-space point, W restored. */
+rc_vec4 rc_project_point(rc_vec4 clip, const rc_matrix *screen);
+/* Reference tiles (1 or 16) a screen triangle is binned to: its TPF share. */
+int64_t rc_tile_refs(const rc_vec4 points[3], int32_t num_tiles, int32_t width, int32_t height);
+
 /* ---- whole frame (renderer.go:443-483, sequential path) ---- */
 int rc_draw(const rc_frame *frame, const rc_mesh *meshes, int32_t num_meshes,
             const rc_texture *textures, int32_t num_textures, const rc_object *objects,

@@ -124,6 +124,9 @@ def load() -> ctypes.CDLL:
         "rc_tile_bounds": (None, [c_int32, c_int32, c_int32, c_int32, POINTER(rc_vec2),
                                   POINTER(rc_vec2)]),
         "rc_fog": (None, [c_int32, c_int32, P, P, ctypes.c_float, ctypes.c_float, P]),
+        "rc_light_intensity": (c_float, [rc_vec4, rc_vec3, c_float, c_float]),
+        "rc_project_point": (rc_vec4, [rc_vec4, P]),
+        "rc_tile_refs": (c_int64, [P, c_int32, c_int32, c_int32]),
         "rc_draw": (c_int32, [POINTER(rc_frame), P, c_int32, P, c_int32, P, c_int32, P, P,
                               POINTER(rc_stats)]),
         "rc_draw_parallel": (c_int32, [POINTER(rc_frame), P, c_int32, P, c_int32, P, c_int32,

@@ -2,7 +2,9 @@
 buffers and its counters for every scene of scene_cases.golden_cases().
 
 Run from the repository root after building the oracle (make -C oracle):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # every scene
+    python tests/golden/make_golden.py NAME ...     # only these (others kept)
+    python tests/golden/make_golden.py --missing    # only scenes without a digest
 The digests pin the oracle and the seeded scene generators against drift; the
 GPU parity tests compare the HIP path with them as well as with the live oracle.
 """
@@ -21,13 +23,29 @@ from scene_cases import golden_cases  # noqa: E402
 
 
 def main():
-    out = {}
-    for name, build in golden_cases():
+    path = os.path.join(HERE, "oracle_digests.json")
+    args = sys.argv[1:]
+    old = json.load(open(path)) if args and os.path.exists(path) else {}
+    cases = golden_cases()
+    names = {n for n, _ in cases}
+    if args == ["--missing"]:
+        todo = names - set(old)
+    elif args:
+        todo = set(args)
+        unknown = todo - names
+        if unknown:
+            raise SystemExit(f"unknown scenes: {sorted(unknown)}")
+    else:
+        todo = names
+    out = {n: d for n, d in old.items() if n in names}
+    for name, build in cases:
+        if name not in todo:
+            continue
         scene = build()
         rgba, depth, st = refcpu.draw(scene)
         out[name] = digest(rgba, depth, st)
         print(name, out[name]["triangles"], out[name]["pixels_shaded"], flush=True)
-    with open(os.path.join(HERE, "oracle_digests.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")
 

@@ -159,6 +159,81 @@ def degenerate(w=64, h=48):
     return S.SceneData(fr, [m], [], [S.ObjectData(0, eye, eye)], name="degenerate")
 
 
+# Hand-derived frames (16x16). mvp = diag(1, 1, 1, -1) puts model vertex
+# (X, Y, Z, Wm) at clip (X, Y, Z, -Wm); with the 16x16 screen matrix the three
+# model vertices (1,1,1), (1,0,1), (0,1,1) land on screen (0,0), (0,8), (8,0).
+# Under the reference's edge functions and top-left rule that triangle covers
+# exactly the 36 pixels with x >= 0, y >= 0, x + y <= 7, at clip W = -1 (depth
+# (a + b) + g, within 2^-22 of 1). It is facing the camera in clip space, and
+# its bounding box is inside the frustum, so it is never clipped. The light is
+# (1, 0, 0), so a world-space normal (nx, 0, 0, nw) lights a face with
+# 0.5 + 0.5 * nx / sqrt(nx^2 + nw^2) (4-D normalisation, vector.go:115-125).
+HAND_MVP = np.diag([1.0, 1.0, 1.0, -1.0]).astype(np.float32)
+HAND_TRI = np.array([[1, 1, 1, 1], [1, 0, 1, 1], [0, 1, 1, 1]], np.float32)
+HAND_FLAGS = (_lib.GR_SHOW_FACES | _lib.GR_BACKFACE_CULLING | _lib.GR_LIGHTING
+              | _lib.GR_FRUSTUM_CLIPPING)
+
+
+def hand_frame(kind="flat-ab"):
+    """Small frames whose every pixel is derived by hand (tests/test_oracle_pins_frame.py):
+
+    flat-ab / flat-ba  two objects, the same triangle and matrices except the
+                       world matrix: A (identity world, face normal (1,0,0,1):
+                       colour uint8(200 * (0.5 + 0.5/sqrt 2)) = 170) and B
+                       (world translated by (2,0,0), so the normal becomes
+                       (3,0,0,1): uint8(200 * (0.5 + 1.5/sqrt 10)) = 194).
+                       Every fragment ties in depth: the later object wins.
+    gouraud            one object lit per corner through normal indices
+                       (1, 1, 1) over normals that differ per vertex; world
+                       translated by (2,0,0): every corner 194.
+    unlit              lighting off, ambient 0.613: uint8(200 * 0.613) = 122.
+    w-zero             the third vertex has clip W = -0: x/w, y/w are NaN or
+                       infinite, the triangle is never binned (background only).
+    z-inf              the third vertex has z/w = -inf with finite x/w, y/w: the
+                       screen matrix's zero terms give 0 * inf = NaN, so the
+                       triangle is never binned.
+    z-big              as z-inf with z/w finite (-2e30): binned and drawn.
+    """
+    fr = S.FrameData(16, 16, flags=HAND_FLAGS, light=np.array([1, 0, 0], np.float32))
+    eye = np.eye(4, dtype=np.float32)
+    shifted = eye.copy()
+    shifted[0, 3] = 2.0
+    f = np.array([[0, 1, 2]], np.int32)
+    fn = np.array([[1, 0, 0, 1]], np.float32)
+    if kind in ("flat-ab", "flat-ba"):
+        fr.flags |= _lib.GR_FLAT_SHADING
+        m = S.MeshData(HAND_TRI.copy(), f, face_normals=fn)
+        a, b = S.ObjectData(0, HAND_MVP, eye), S.ObjectData(0, HAND_MVP, shifted)
+        objs = [a, b] if kind == "flat-ab" else [b, a]
+        return S.SceneData(fr, [m], [], objs, name=f"hand-{kind}")
+    if kind == "gouraud":
+        vn = np.array([[7, 0, 0, 1], [1, 0, 0, 1], [0, 1, 0, 1]], np.float32)
+        m = S.MeshData(HAND_TRI.copy(), f, vertex_normals=vn,
+                       normal_indices=np.array([[1, 1, 1]], np.int32), face_normals=fn)
+        return S.SceneData(fr, [m], [], [S.ObjectData(0, HAND_MVP, shifted)],
+                           name="hand-gouraud")
+    if kind == "unlit":
+        fr.flags &= ~_lib.GR_LIGHTING
+        fr.ambient = 0.613
+        m = S.MeshData(HAND_TRI.copy(), f, face_normals=fn)
+        return S.SceneData(fr, [m], [], [S.ObjectData(0, HAND_MVP, eye)], name="hand-unlit")
+    v = HAND_TRI.copy()
+    fr.flags &= ~(_lib.GR_BACKFACE_CULLING | _lib.GR_FRUSTUM_CLIPPING)
+    if kind == "w-zero":
+        v[2] = [0, 1, 1, 0]
+    elif kind == "z-inf":
+        v[2] = [0, 1, 3e38, 0.5]
+    elif kind == "z-big":
+        v[2] = [0, 1, 1e30, 0.5]
+    else:
+        raise ValueError(kind)
+    m = S.MeshData(v, f, face_normals=fn)
+    return S.SceneData(fr, [m], [], [S.ObjectData(0, HAND_MVP, eye)], name=f"hand-{kind}")
+
+
+HAND_KINDS = ("flat-ab", "flat-ba", "gouraud", "unlit", "w-zero", "z-inf", "z-big")
+
+
 def all_cases():
     """(name, builder) pairs of the small parity suite."""
     return [
@@ -200,7 +275,7 @@ def all_cases():
         ("overlay-ties", lambda: with_flags(overlapping_objects(), ALL | _lib.GR_SHOW_EDGES)),
         ("overlay-inside", lambda: inside_sphere(flags=ALL | _lib.GR_SHOW_EDGES
                                                  | _lib.GR_SHOW_VERTICES)),
-    ]
+    ] + [(f"hand-{k}", (lambda k=k: hand_frame(k))) for k in HAND_KINDS]
 
 
 @functools.lru_cache(maxsize=1)

@@ -1,0 +1,316 @@
+"""Pins of the oracle's geometry stage and whole frame, and of the product's
+host matrix helpers, against what the mathematics fixes:
+
+* Lighting (renderer.go:312-346): the world-space normal is normalised in 4-D
+  with its W (vector.go:115-125), so axis normals with W = 1 give the closed
+  forms 0.5 + 0.5 * n / sqrt(n^2 + 1); a 3-D normalisation, a dropped W or a
+  swapped ambient/diffuse term each give a different value.
+* Divide + screen (renderer.go:361-370, matrix.go:167-173): exact quotients,
+  the W restore, and the IEEE 0 * inf = NaN of the screen matrix's zero terms.
+* TPF (renderer.go:226-244,436-441): tile sets counted by hand on the 4x4
+  grid, shared inclusive borders included.
+* Whole frames (renderer.go:443-474): 16x16 scenes whose every pixel, depth
+  and counter is derived by hand (tests/scene_cases.py hand_frame).
+* gr_matrix_* of the product library against the oracle's builders, bit for
+  bit, and against closed forms.
+"""
+import ctypes
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from gorender_amd import _lib
+from gorender_amd import scenes as S
+from oracle import refcpu
+from oracle.refcpu import _p, rc_vec3, rc_vec4
+from scene_cases import hand_frame
+
+F32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return refcpu.load()
+
+
+def rn(x):
+    """Round an exact rational (or float) to the nearest float32."""
+    return Fraction(float(F32(float(x)))) if not isinstance(x, Fraction) else _rn_frac(x)
+
+
+def _rn_frac(q):
+    # float(q) rounds correctly to float64; float32 of that double can double-round
+    # only on exact float64 midpoints, which these operands never reach.
+    return Fraction(float(F32(float(q))))
+
+
+def ulps32(a, b):
+    ia = np.array([a], F32).view(np.int32)[0]
+    ib = np.array([b], F32).view(np.int32)[0]
+    return abs(int(ia) - int(ib))
+
+
+# ---------------------------------------------------------------------------
+# Lighting
+# ---------------------------------------------------------------------------
+
+def light(lib, n, L=(1.0, 0.0, 0.0), ambient=0.5, diffuse=0.5):
+    return lib.rc_light_intensity(rc_vec4(*n), rc_vec3(*L), ambient, diffuse)
+
+
+def test_light_4d_normalisation_closed_forms(lib):
+    # (nx, 0, 0, 1) with the light along x: 0.5 + 0.5 * nx / sqrt(nx^2 + 1).
+    for nx in (1.0, 2.0, 3.0, 0.5, -1.0, 7.0):
+        want = 0.5 + 0.5 * nx / math.sqrt(nx * nx + 1.0)
+        got = light(lib, (nx, 0.0, 0.0, 1.0))
+        assert ulps32(got, want) <= 2, (nx, got, want)
+    # A 3-D normalisation would give 1.0 for every positive nx: not the case.
+    assert light(lib, (1.0, 0.0, 0.0, 1.0)) < 0.86
+    # (3, 0, 0, 4): length 5 exactly, x = RN(3/5), I = RN(0.5 + x/2).
+    x = rn(Fraction(3, 5))
+    assert light(lib, (3.0, 0.0, 0.0, 4.0)) == float(rn(Fraction(1, 2) + x / 2))
+    # W alone: the normal has no x, y, z after normalisation -> ambient exactly.
+    assert light(lib, (0.0, 0.0, 0.0, 1.0)) == 0.5
+    # Light along z and a z normal: the dot product picks the normal's Z.
+    assert ulps32(light(lib, (0.0, 0.0, 1.0, 1.0), L=(0, 0, 1)), 0.5 + 0.5 / math.sqrt(2)) <= 2
+    # A zero normal normalises to NaN (0/0), and so does the intensity.
+    assert math.isnan(light(lib, (0.0, 0.0, 0.0, 0.0)))
+
+
+def test_light_ambient_and_diffuse_terms(lib):
+    # ambient + dot * diffuse, not (ambient + dot) * diffuse or dot + ambient * diffuse:
+    # nx = 3, nw = 4 -> dot = RN(0.6).
+    d = rn(Fraction(3, 5))
+    got = light(lib, (3.0, 0.0, 0.0, 4.0), ambient=0.25, diffuse=0.75)
+    assert got == float(rn(Fraction(1, 4) + rn(d * Fraction(3, 4))))
+    assert abs(got - (0.25 + 0.6 * 0.75)) < 1e-6
+    # Light opposite the normal: the diffuse term is subtracted.
+    assert ulps32(light(lib, (1.0, 0.0, 0.0, 0.0), L=(-1, 0, 0)), 0.0) == 0
+
+
+def test_default_light_direction():
+    # Vec3{-1, 1, 1}.Normalize() (renderer.go:265): each component +-1/RN(sqrt 3).
+    fr = S.FrameData(4, 4)
+    s3 = float(F32(math.sqrt(3.0)))
+    want = np.array([-1.0, 1.0, 1.0], np.float64) / s3
+    assert np.array_equal(np.asarray(fr.light, F32), want.astype(F32))
+
+
+# ---------------------------------------------------------------------------
+# Perspective divide and screen transform
+# ---------------------------------------------------------------------------
+
+def project(lib, clip, w=640, h=360):
+    scr = refcpu.matrix(S.screen_matrix(w, h))
+    p = lib.rc_project_point(rc_vec4(*clip), ctypes.byref(scr))
+    return (p.x, p.y, p.z, p.w)
+
+
+def test_project_exact_quotients_and_w_restore(lib):
+    # 640x360 screen: x' = 320 * x/w + 320, y' = 180 * y/w + 180, z' = z/(2w) + 0.5.
+    assert project(lib, (2.0, -1.0, 0.5, 2.0)) == (640.0, 90.0, 0.625, 2.0)
+    assert project(lib, (-3.0, 1.5, -6.0, -3.0)) == (640.0, 90.0, 1.5, -3.0)
+    assert project(lib, (1.0, 1.0, 1.0, -4.0)) == (240.0, 135.0, 0.375, -4.0)
+    # A non-exact quotient: x/w = RN(1/3), then RN(320 * q) + 320 rounded once more.
+    x, y, z, w = project(lib, (1.0, 0.0, 0.0, 3.0))
+    q = rn(Fraction(1, 3))
+    assert x == float(rn(rn(320 * q) + 320)) and w == 3.0
+    assert abs(x - (320 / 3 + 320)) < 1e-4
+
+
+def test_project_w_zero_and_infinite_depth(lib):
+    # W = 0: x/0 is infinite and w/w NaN, so x', y' are NaN; W stays 0.
+    x, y, z, w = project(lib, (1.0, 2.0, 3.0, 0.0))
+    assert math.isnan(x) and math.isnan(y) and w == 0.0
+    # z/w = -inf with finite x/w, y/w: the zero terms of the screen matrix's x
+    # and y rows multiply it (0 * inf = NaN), so x' and y' are NaN too.
+    x, y, z, w = project(lib, (0.0, 1.0, 3e38, -0.5))
+    assert math.isnan(x) and math.isnan(y) and z == -math.inf and w == -0.5
+    # z/w large but finite: the zero terms give -0 and the point is finite.
+    x, y, z, w = project(lib, (0.0, 1.0, 1e30, -0.5))
+    assert (x, y) == (320.0, -180.0) and math.isfinite(z)
+
+
+# ---------------------------------------------------------------------------
+# TPF on the reference grid
+# ---------------------------------------------------------------------------
+
+def tile_refs(lib, pts, tiles=16, w=640, h=360):
+    p = np.array([[x, y, 0.0, -1.0] for x, y in pts], F32)
+    return lib.rc_tile_refs(_p(p), tiles, w, h)
+
+
+def test_tpf_hand_counted(lib):
+    # 640x360 on the 4x4 grid: columns [0,160] [160,320] [320,480] [480,640],
+    # rows [0,90] [90,180] [180,270] [270,360], borders shared (inclusive).
+    cases = [
+        ([(10, 10), (20, 10), (10, 20)], 1),        # inside tile 0
+        ([(150, 80), (170, 80), (150, 100)], 4),    # straddles a corner
+        ([(100, 10), (160, 10), (100, 20)], 2),     # max x on the shared border
+        ([(160, 10), (200, 10), (160, 20)], 2),     # min x on the shared border
+        ([(161, 10), (200, 10), (161, 20)], 1),
+        ([(-50, -50), (-10, -50), (-50, -10)], 0),  # off screen
+        ([(0, 0), (640, 0), (0, 360)], 16),         # the whole grid
+        ([(-10, 100), (650, 100), (-10, 100)], 4),  # one row, every column
+        ([(640, 360), (700, 400), (650, 380)], 1),  # touches the last corner only
+        ([(float("nan"), 10), (20, 10), (10, 20)], 0),  # Go min/max propagate NaN
+    ]
+    for pts, want in cases:
+        assert tile_refs(lib, pts) == want, pts
+    # One reference tile (sequential mode): the full-screen rectangle.
+    assert tile_refs(lib, [(640, 10), (700, 10), (640, 20)], tiles=1) == 1
+    assert tile_refs(lib, [(641, 10), (700, 10), (641, 20)], tiles=1) == 0
+    # Sizes that do not divide: 642 wide -> columns of 161, the last [483, 642].
+    assert tile_refs(lib, [(483, 10), (500, 10), (483, 20)], w=642) == 2
+    assert tile_refs(lib, [(484, 10), (500, 10), (484, 20)], w=642) == 1
+
+
+# ---------------------------------------------------------------------------
+# Whole frames derived by hand
+# ---------------------------------------------------------------------------
+
+HAND_COVER = np.array([[x + y <= 7 for x in range(16)] for y in range(16)])  # [y][x]
+
+
+def background(dot=True):
+    img = np.zeros((16, 16, 4), np.uint8)
+    img[:] = (50, 50, 50, 255)
+    if dot:
+        img[10, 10] = (100, 100, 100, 255)  # DotGrid step 10 on a 16x16 frame
+    return img
+
+
+def check_hand(scene, colour, tpf, triangles, covered):
+    rgba, depth, st = refcpu.draw(scene)
+    want = background()
+    if colour is not None:
+        want[HAND_COVER] = colour
+    assert np.array_equal(rgba, want)
+    assert np.all(depth[~HAND_COVER] == -1.0) if colour is not None else np.all(depth == -1.0)
+    if colour is not None:
+        # clip W = -1 at every vertex: depth = (a + b) + g, within 2^-22 of 1.
+        assert np.all(np.abs(depth[HAND_COVER].astype(np.float64) - 1.0) <= 2.0 ** -22)
+    assert st["tpf"] == tpf and st["triangles"] == triangles
+    assert st["fragments_covered"] == covered
+    return rgba, depth, st
+
+
+def test_hand_frame_flat_two_objects_later_wins():
+    # A: face normal (1,0,0,1), identity world -> 0.5 + 0.5/sqrt 2 -> 170.
+    # B: the world translates by (2,0,0), which the normal picks up with W = 1:
+    #    (3,0,0,1) -> 0.5 + 1.5/sqrt 10 -> 194. Equal depths: the later wins.
+    # TPF: the float box [0,8]^2 overlaps columns [0,4] [4,8] [8,12] and the
+    # same rows: 9 tiles per triangle.
+    assert int(200 * (0.5 + 0.5 / math.sqrt(2))) == 170
+    assert int(200 * (0.5 + 1.5 / math.sqrt(10))) == 194
+    _, d_ab, _ = check_hand(hand_frame("flat-ab"), (194, 194, 194, 255), 18, 2, 72)
+    _, d_ba, _ = check_hand(hand_frame("flat-ba"), (170, 170, 170, 255), 18, 2, 72)
+    assert np.array_equal(d_ab.view(np.uint32), d_ba.view(np.uint32))
+
+
+def test_hand_frame_gouraud_uses_normal_indices():
+    # Normal indices (1,1,1) select (1,0,0,1) at every corner, moved by the
+    # world translation to (3,0,0,1): 194 everywhere (the vertex indices
+    # (0,1,2) would select three different normals).
+    check_hand(hand_frame("gouraud"), (194, 194, 194, 255), 9, 1, 36)
+
+
+def test_hand_frame_unlit_is_ambient():
+    check_hand(hand_frame("unlit"), (122, 122, 122, 255), 9, 1, 36)
+
+
+def test_hand_frame_nan_and_infinite_points_are_not_binned():
+    check_hand(hand_frame("w-zero"), None, 0, 0, 0)
+    check_hand(hand_frame("z-inf"), None, 0, 0, 0)
+    # z/w finite: the same triangle is binned and covers pixels.
+    rgba, depth, st = refcpu.draw(hand_frame("z-big"))
+    assert st["triangles"] == 1 and st["fragments_covered"] > 0
+
+
+# ---------------------------------------------------------------------------
+# The product's host matrix helpers against the oracle's builders
+# ---------------------------------------------------------------------------
+
+def _gr(fn, *args):
+    out = np.zeros((4, 4), F32)
+    fn(*args, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def _v3(a):
+    return np.ascontiguousarray(a, F32)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, F32).view(np.uint32)
+
+
+def test_host_matrices_equal_oracle_builders_bit_for_bit(lib):
+    g = _lib.load()
+    rng = np.random.default_rng(20261203)
+    for _ in range(300):
+        s, r, t = (rng.uniform(-3, 3, 3).astype(F32) for _ in range(3))
+        r[rng.random(3) < 0.2] = 0.0  # angle 0 takes the identity short cut
+        want = refcpu.matrix_np(lib.rc_world_matrix(rc_vec3(*s), rc_vec3(*r), rc_vec3(*t)))
+        got = _gr(g.gr_matrix_world, _p(_v3(s)), _p(_v3(r)), _p(_v3(t)))
+        assert np.array_equal(_bits(got), _bits(want))
+
+        eye, d, up = (rng.uniform(-5, 5, 3).astype(F32) for _ in range(3))
+        want = refcpu.matrix_np(lib.rc_view_matrix(rc_vec3(*eye), rc_vec3(*d), rc_vec3(*up)))
+        got = _gr(g.gr_matrix_view, _p(_v3(eye)), _p(_v3(d)), _p(_v3(up)))
+        assert np.array_equal(_bits(got), _bits(want))
+
+        fov = float(F32(rng.uniform(0.2, 2.5)))
+        aspect = float(F32(rng.uniform(0.3, 3.0)))
+        zn = float(F32(rng.choice([0.0, rng.uniform(0.01, 1.0)])))
+        zf = float(F32(rng.uniform(2.0, 200.0)))
+        want = refcpu.matrix_np(lib.rc_perspective_matrix(fov, aspect, zn, zf))
+        got = _gr(g.gr_matrix_perspective, fov, aspect, zn, zf)
+        assert np.array_equal(_bits(got), _bits(want))
+
+        w, h = int(rng.integers(1, 4000)), int(rng.integers(1, 4000))
+        want = refcpu.matrix_np(lib.rc_screen_matrix(w, h))
+        assert np.array_equal(_bits(_gr(g.gr_matrix_screen, w, h)), _bits(want))
+
+        a, b, c = (rng.uniform(-2, 2, (4, 4)).astype(F32) for _ in range(3))
+        ma, mb, mc = refcpu.matrix(a), refcpu.matrix(b), refcpu.matrix(c)
+        ident = refcpu.matrix(np.eye(4, dtype=F32))
+        # mvp = ((I * P) * V) * W (renderer.go:259-262)
+        want = lib.rc_matrix_multiply(ctypes.byref(ident), ctypes.byref(ma))
+        want = lib.rc_matrix_multiply(ctypes.byref(want), ctypes.byref(mb))
+        want = refcpu.matrix_np(lib.rc_matrix_multiply(ctypes.byref(want), ctypes.byref(mc)))
+        got = _gr(g.gr_matrix_mvp, _p(a), _p(b), _p(c))
+        assert np.array_equal(_bits(got), _bits(want))
+        want2 = refcpu.matrix_np(lib.rc_matrix_multiply(ctypes.byref(ma), ctypes.byref(mb)))
+        assert np.array_equal(_bits(_gr(g.gr_matrix_multiply, _p(a), _p(b))), _bits(want2))
+
+
+def test_host_matrices_closed_forms():
+    g = _lib.load()
+    # Angle 0 short-cuts to the identity (matrix.go:33-35): world = T * S exactly.
+    w = _gr(g.gr_matrix_world, _p(_v3([2, 3, 4])), _p(_v3([0, 0, 0])), _p(_v3([5, -6, 7])))
+    assert np.array_equal(w, np.array([[2, 0, 0, 5], [0, 3, 0, -6], [0, 0, 4, 7], [0, 0, 0, 1]],
+                                      F32))
+    # Axis-aligned view from (0,0,3) along -z with y up: rows x = (-1,0,0),
+    # y = (0,1,0), z = (0,0,-1), translation -axis . eye.
+    v = _gr(g.gr_matrix_view, _p(_v3([0, 0, 3])), _p(_v3([0, 0, -1])), _p(_v3([0, 1, 0])))
+    assert np.array_equal(v, np.array([[-1, 0, 0, 0], [0, 1, 0, 0], [0, 0, -1, 3],
+                                       [0, 0, 0, 1]], F32))
+    # fov = float32(pi/2): tan(fov/2) rounds to 1.0 in float32 (4.4e-8 from 1,
+    # under half an ulp), so m00 = 1/aspect, m11 = 1; zNear 0, zFar 50 give
+    # m22 = 50/-50 = -1 and m23 = 0/-50 = -0, stored negated (matrix.go:92-106).
+    p = _gr(g.gr_matrix_perspective, float(F32(math.pi / 2)), 2.0, 0.0, 50.0)
+    want = np.array([[0.5, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, -1, 0]], F32)
+    assert np.array_equal(p, want)
+    assert np.signbit(p[2, 3]) == False  # -(-0) = +0
+    # Screen 640x360: half sizes on the diagonal and in the last column.
+    s = _gr(g.gr_matrix_screen, 640, 360)
+    assert np.array_equal(s, np.array([[320, 0, 0, 320], [0, 180, 0, 180], [0, 0, 0.5, 0.5],
+                                       [0, 0, 0, 1]], F32))
+    # A rotation by pi/2 about z maps x to y up to the float32 rounding of cos.
+    r = _gr(g.gr_matrix_world, _p(_v3([1, 1, 1])), _p(_v3([0, 0, float(F32(math.pi / 2))])),
+            _p(_v3([0, 0, 0])))
+    assert abs(r[0, 0]) < 1e-7 and abs(r[1, 1]) < 1e-7
+    assert abs(abs(r[0, 1]) - 1) < 1e-7 and abs(abs(r[1, 0]) - 1) < 1e-7
