@@ -129,6 +129,8 @@ SIGNATURES = {
     "gr_draw_device": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32, c_void_p,
                                  c_void_p]),
     "gr_draw_batch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32]),
+    "gr_draw_device_noalloc": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32,
+                                         c_void_p, c_void_p]),
     "gr_synchronize": (c_int32, [c_void_p]),
     "gr_last_stats": (c_int32, [c_void_p, c_void_p, c_int32, POINTER(c_int32)]),
     "gr_device_alloc": (c_int32, [c_void_p, ctypes.c_size_t, POINTER(c_void_p)]),

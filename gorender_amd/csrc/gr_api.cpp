@@ -108,6 +108,9 @@ struct gr_context {
     void* h_ctrl = nullptr;   // pinned: sticky control words
     int32_t last_frames = 0;
     const unsigned long long* last_stats = nullptr;  // device stats of the last draw
+    // Control words of the no-allocation draw (in its fixed device array),
+    // once this context has used it.
+    gr::DevControl* noalloc_ctrl = nullptr;
 
     gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 0};
 
@@ -334,6 +337,12 @@ bool dense_on_screen(const float* mvp, const float bbox[8][4], int F, int W, int
 // Device memory for the depth keys of split tiles.
 constexpr unsigned long long kSlabBudget = 64ull << 20;
 
+// Tile-list entries per batch: offsets and cursors are 32-bit.
+constexpr int64_t kMaxRefs = 0xFFFFFFFFll;
+
+// Objects of one no-allocation draw (their descriptors are built on the stack).
+constexpr int kNoAllocMaxObjects = 16;
+
 // Everything that makes one frame of a batch invalid, checked before any
 // launch. The serving loop runs it when a request is submitted, so one bad
 // request cannot fail a batch it would share.
@@ -364,20 +373,38 @@ gr_status check_frame(gr_context* c, const gr_frame& fr, const gr_object* objs, 
     return GR_OK;
 }
 
-// The whole pipeline for a batch of frames. Objects must be laid out frame by
-// frame; outputs are device pointers.
-gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
-                  int no) {
-    if (nf <= 0) return fail(GR_ERR_INVALID_ARGUMENT, "no frames");
-    const int TW = c->cfg.tile_w, TH = c->cfg.tile_h;
-    const int twl = ilog2(TW), thl = ilog2(TH);
-
-    std::vector<gr::DevFrame> frames(nf);
-    std::vector<gr::DevObject> dobj;
-    dobj.reserve(no);
-    int fblocks = 0, tiles = 0;
+// Host-side description of one batch: the device structs (buffer offsets in
+// place of buffer pointers), the header layout and the sizes of the per-batch
+// buffers. Objects must be laid out frame by frame.
+struct Plan {
+    int nf = 0, nobj = 0, fblocks = 0, tiles = 0;
+    int twl = 0, thl = 0;
     size_t slots = 0, okey_words = 0;
     bool decorate = false;  // a frame asks for tile borders or the crosshair
+    gr::DevFrame* frames = nullptr;  // [nf], caller storage
+    gr::DevObject* dobj = nullptr;   // [nobj], caller storage
+    int* xcd = nullptr;              // [8 * (nf + 1)], caller storage
+    int face_grid = 0;
+    size_t off_objs = 0, off_stats = 0, off_counters = 0, off_over = 0, off_xcd = 0;
+    size_t header_bytes = 0;
+};
+
+size_t xcd_ints(int nf) { return 8 * ((size_t)nf + 1); }
+
+// Validates the batch and fills the plan. `frames`, `dobj` and `xcd` hold at
+// least nf, no and xcd_ints(nf) entries.
+gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
+                    int no, int TW, int TH, gr::DevFrame* frames, gr::DevObject* dobj, int* xcd,
+                    Plan& P) {
+    if (nf <= 0) return fail(GR_ERR_INVALID_ARGUMENT, "no frames");
+    P = Plan{};
+    P.nf = nf;
+    P.twl = ilog2(TW);
+    P.thl = ilog2(TH);
+    P.frames = frames;
+    P.dobj = dobj;
+    P.xcd = xcd;
+    int nobj = 0;
     for (int f = 0; f < nf; f++) {
         const gr_frame& fr = bf[f].frame;
         if (bf[f].num_objects < 0 || bf[f].first_object < 0 ||
@@ -407,25 +434,25 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             ref_tile_bounds(t, fr.ref_tiles, fr.width, fr.height, d.ref_tile_rect[t]);
         d.tiles_x = (fr.width + TW - 1) / TW;
         d.tiles_y = (fr.height + TH - 1) / TH;
-        d.tile_begin = tiles;
-        tiles += d.tiles_x * d.tiles_y;
-        d.face_block_begin = fblocks;
+        d.tile_begin = P.tiles;
+        P.tiles += d.tiles_x * d.tiles_y;
+        d.face_block_begin = P.fblocks;
         d.rgba = (uint8_t*)bf[f].d_rgba;
         d.depth = (float*)bf[f].d_depth;
         // Overlay key buffer offset (bytes + 1; 0 = no overlays).
-        if (fr.flags & (GR_SHOW_TILE_BOUNDS | GR_SHOW_CROSSHAIR)) decorate = true;
+        if (fr.flags & (GR_SHOW_TILE_BOUNDS | GR_SHOW_CROSSHAIR)) P.decorate = true;
         d.okey = nullptr;
         if (fr.flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES)) {
-            d.okey = (uint32_t*)(uintptr_t)(okey_words * 4 + 1);
-            okey_words += (size_t)fr.width * fr.height;
+            d.okey = (uint32_t*)(uintptr_t)(P.okey_words * 4 + 1);
+            P.okey_words += (size_t)fr.width * fr.height;
         }
         int seq = 0;
-        d.obj_begin = (int32_t)dobj.size();
+        d.obj_begin = nobj;
         d.obj_count = bf[f].num_objects;
         for (int k = 0; k < bf[f].num_objects; k++) {
             const gr_object& ob = objs[bf[f].first_object + k];
             const MeshRes& mr = c->meshes[ob.mesh];
-            gr::DevObject o;
+            gr::DevObject& o = dobj[nobj++];
             std::memset(&o, 0, sizeof o);
             o.m = mr.dm;
             std::memcpy(o.mvp, ob.mvp, sizeof o.mvp);
@@ -434,174 +461,263 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
             o.gouraud = (fr.flags & GR_LIGHTING) && !(fr.flags & GR_FLAT_SHADING) && mr.dm.N > 0;
             o.dense = dense_on_screen(ob.mvp, mr.dm.bbox, mr.dm.F, fr.width, fr.height) ? 1 : 0;
             o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
-            o.face_block = fblocks;
+            o.face_block = P.fblocks;
             o.seq_block = seq;
-            fblocks += o.face_blocks;
+            P.fblocks += o.face_blocks;
             seq += o.face_blocks;
-            dobj.push_back(o);
         }
-        d.face_blocks = fblocks - d.face_block_begin;
-        d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)(slots * sizeof(gr::OverlayRec) + 1) : nullptr;
-        d.rrec = (gr::RasterRec*)(uintptr_t)(slots * sizeof(gr::RasterRec));
-        d.srec = (gr::ShadeRec*)(uintptr_t)(slots * sizeof(gr::ShadeRec));
-        d.wrec = (gr::WideRec*)(uintptr_t)(slots * sizeof(gr::WideRec));
-        slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
+        d.face_blocks = P.fblocks - d.face_block_begin;
+        d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)(P.slots * sizeof(gr::OverlayRec) + 1) : nullptr;
+        d.rrec = (gr::RasterRec*)(uintptr_t)(P.slots * sizeof(gr::RasterRec));
+        d.srec = (gr::ShadeRec*)(uintptr_t)(P.slots * sizeof(gr::ShadeRec));
+        d.wrec = (gr::WideRec*)(uintptr_t)(P.slots * sizeof(gr::WideRec));
+        P.slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
     }
-    const int nobj = (int)dobj.size();
+    P.nobj = nobj;
+    P.off_objs = align256((size_t)nf * sizeof(gr::DevFrame));
+    P.off_stats = P.off_objs + align256((size_t)nobj * sizeof(gr::DevObject));
+    P.off_counters = P.off_stats + align256((size_t)nf * gr::kNumStats * 8);
+    P.off_over = P.off_counters + align256(sizeof(gr::DevBatchCounters));
+    P.off_xcd = P.off_over + align256((size_t)nf * 4);
+    P.header_bytes = P.off_xcd + align256(xcd_ints(nf) * sizeof(int));
+    int nface[64];
+    std::vector<int> nface_big;
+    int* nfp = nface;
+    if (nf > 64) {
+        nface_big.resize(nf);
+        nfp = nface_big.data();
+    }
+    for (int f = 0; f < nf; f++) nfp[f] = frames[f].face_blocks;
+    gr::xcd_tables(nfp, nf, xcd, &P.face_grid);
+    return GR_OK;
+}
+
+// Device buffers of one batch: a pipe's, a batch graph's, or carved from the
+// fixed scratch array of the no-allocation draw.
+struct Bufs {
+    char* header = nullptr;  // frames | objects | stats | counters | frame_overflow | XCD tables
+    gr::RasterRec* rrec = nullptr;
+    gr::ShadeRec* srec = nullptr;
+    gr::WideRec* wrec = nullptr;
+    uint32_t* tile_counts = nullptr;  // zero between draws (k_raster re-zeroes)
+    uint32_t* tiles = nullptr;        // offsets | cursors | slabs
+    gr::RasterItem* items = nullptr;
+    uint32_t max_items = 0;
+    unsigned long long* slabs = nullptr;  // zero between draws
+    uint32_t max_slabs = 0;
+    uint32_t part_refs = 0;
+    uint32_t* wgcount = nullptr;
+    uint32_t* refs = nullptr;
+    unsigned long long refs_cap = 0;
+    gr::DevControl* ctrl = nullptr;  // sticky until read
+    uint32_t* okey = nullptr;        // zero between draws
+    gr::OverlayRec* orec = nullptr;
+    const gr::DevTexture* textures = nullptr;
+};
+
+// Split tiles: a tile is split only when its list is longer than part_refs,
+// so the items of a batch number at most refs_cap / part_refs and the split
+// tiles at most refs_cap / (part_refs + 1); slabs are further capped by a
+// memory budget (a tile without a slab is rasterised whole).
+void split_limits(unsigned long long refs_cap, uint32_t part_refs, size_t slab_bytes,
+                  uint32_t* max_items, uint32_t* max_slabs) {
+    *max_items = (uint32_t)std::min<unsigned long long>(refs_cap / part_refs + 1, 0xFFFFFFF0ull);
+    *max_slabs = (uint32_t)std::max<unsigned long long>(
+        1, std::min<unsigned long long>(refs_cap / (part_refs + 1ull) + 1, kSlabBudget / slab_bytes));
+}
+
+// The header as uploaded: device pointers patched in, counters zero.
+void write_header(const Plan& P, const Bufs& B, void* h) {
+    std::memset(h, 0, P.header_bytes);
+    gr::DevFrame* hf = (gr::DevFrame*)h;
+    std::memcpy(hf, P.frames, (size_t)P.nf * sizeof(gr::DevFrame));
+    for (int f = 0; f < P.nf; f++) {
+        gr::DevFrame& d = hf[f];
+        if (d.okey) d.okey = (uint32_t*)((char*)B.okey + ((uintptr_t)d.okey - 1));
+        if (d.orec) d.orec = (gr::OverlayRec*)((char*)B.orec + ((uintptr_t)d.orec - 1));
+        d.ctrl = B.ctrl;
+        d.rrec = (gr::RasterRec*)((char*)B.rrec + (uintptr_t)d.rrec);
+        d.srec = (gr::ShadeRec*)((char*)B.srec + (uintptr_t)d.srec);
+        d.wrec = (gr::WideRec*)((char*)B.wrec + (uintptr_t)d.wrec);
+    }
+    if (P.nobj) std::memcpy((char*)h + P.off_objs, P.dobj, (size_t)P.nobj * sizeof(gr::DevObject));
+    std::memcpy((char*)h + P.off_xcd, P.xcd, xcd_ints(P.nf) * sizeof(int));
+}
+
+gr::LaunchParams launch_params(const Plan& P, const Bufs& B) {
+    char* bb = B.header;
+    gr::LaunchParams p{};
+    p.objs = (const gr::DevObject*)(bb + P.off_objs);
+    p.nobj = P.nobj;
+    p.frames = (const gr::DevFrame*)bb;
+    p.nframes = P.nf;
+    p.textures = B.textures;
+    p.face_blocks = P.fblocks;
+    p.total_tiles = P.tiles;
+    p.tw_log2 = P.twl;
+    p.th_log2 = P.thl;
+    p.tile_counts = B.tile_counts;
+    p.tile_offsets = B.tiles;
+    p.tile_cursor = B.tiles + P.tiles;
+    p.tile_slab = B.tiles + 2 * (size_t)P.tiles;
+    p.items = B.items;
+    p.max_items = B.max_items;
+    p.slabs = B.slabs;
+    p.max_slabs = B.max_slabs;
+    p.part_refs = B.part_refs;
+    p.frame_overflow = (uint32_t*)(bb + P.off_over);
+    p.wg_count = B.wgcount;
+    p.refs = B.refs;
+    p.refs_capacity = B.refs_cap;
+    p.stats = (unsigned long long*)(bb + P.off_stats);
+    p.counters = (gr::DevBatchCounters*)(bb + P.off_counters);
+    p.ctrl = B.ctrl;
+    p.xcd_face = (const int*)(bb + P.off_xcd);
+    p.face_grid = P.face_grid;
+    return p;
+}
+
+// The kernels of one batch on stream s. `raster_after` (may be null): an event
+// the raster waits for, so that frames are written in submission order.
+gr_status launch_plan(gr_context* c, const Plan& P, const gr::LaunchParams& p, hipStream_t s,
+                      hipEvent_t raster_after) {
+    timed_launch(c, "k_face", s, [&] { gr::launch_face(p, s); });
+    timed_launch(c, "k_bin_scan", s, [&] { gr::launch_bin_scan(p, s); });
+    timed_launch(c, "k_bin_fill", s, [&] { gr::launch_bin_fill(p, s); });
+    if (P.okey_words) timed_launch(c, "k_overlay", s, [&] { gr::launch_overlay(p, s); });
+    bool ok = true;
+    timed_launch(c, "k_raster_parts", s, [&] { ok = gr::launch_raster_parts(p, s); });
+    if (raster_after) GR_HIP(hipStreamWaitEvent(s, raster_after, 0));
+    if (ok) timed_launch(c, "k_raster", s, [&] { ok = gr::launch_raster(p, s); });
+    // Drawn over the finished frame, before the event the next batch's raster
+    // waits for (it may write the same buffers).
+    if (ok && P.decorate) timed_launch(c, "k_decorate", s, [&] { gr::launch_decorate(p, s); });
+    hipError_t le = hipGetLastError();
+    if (!ok || le != hipSuccess) {
+        // Leave the tile counters zeroed for the next draw.
+        gr::launch_fill32(p.tile_counts, 0u, (size_t)P.tiles, s);
+        if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
+        return fail(GR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(le));
+    }
+    return GR_OK;
+}
+
+// Allocates (or grows) buffer b and zero-fills a new allocation on stream s.
+gr_status ensure_zeroed(gr_context* c, DevBuf& b, size_t bytes, hipStream_t s) {
+    if (b.cap >= bytes && b.p) return GR_OK;
+    gr_status st = ensure(c, b, bytes);
+    if (st) return st;
+    gr::launch_fill32((uint32_t*)b.p, 0u, b.cap / 4, s);
+    GR_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+// Device buffers of a plan in a set of growable buffers (a pipe's or a batch
+// graph's); new allocations that must start at zero are zero-filled on s.
+gr_status size_bufs(gr_context* c, const Plan& P, Pipe& Q, unsigned long long refs_cap,
+                    uint32_t part_refs, int TW, int TH, hipStream_t s, Bufs& B) {
+    gr_status st;
+    if ((st = ensure(c, Q.batch, P.header_bytes))) return st;
+    if ((st = ensure(c, Q.rrec, P.slots * sizeof(gr::RasterRec)))) return st;
+    if ((st = ensure(c, Q.srec, P.slots * sizeof(gr::ShadeRec)))) return st;
+    if ((st = ensure(c, Q.wrec, P.slots * sizeof(gr::WideRec)))) return st;
+    if ((st = ensure_zeroed(c, Q.tile_counts, (size_t)P.tiles * 4, s))) return st;
+    if ((st = ensure(c, Q.tiles, 3 * (size_t)P.tiles * sizeof(uint32_t)))) return st;
+    const size_t slab_bytes = (size_t)TW * TH * 8;
+    uint32_t max_items, max_slabs;
+    split_limits(refs_cap, part_refs, slab_bytes, &max_items, &max_slabs);
+    if ((st = ensure(c, Q.items, (size_t)max_items * sizeof(gr::RasterItem)))) return st;
+    if ((st = ensure_zeroed(c, Q.slabs, (size_t)max_slabs * slab_bytes, s))) return st;
+    if ((st = ensure(c, Q.wgcount, std::max(P.fblocks, 1) * sizeof(uint32_t)))) return st;
+    if ((st = ensure(c, Q.refs, refs_cap * sizeof(uint32_t)))) return st;
+    if ((st = ensure_zeroed(c, Q.ctrl, sizeof(gr::DevControl), s))) return st;
+    if (P.okey_words) {
+        if ((st = ensure(c, Q.orec, P.slots * sizeof(gr::OverlayRec)))) return st;
+        if ((st = ensure_zeroed(c, Q.okey, P.okey_words * 4, s))) return st;
+    }
+    B.header = (char*)Q.batch.p;
+    B.rrec = (gr::RasterRec*)Q.rrec.p;
+    B.srec = (gr::ShadeRec*)Q.srec.p;
+    B.wrec = (gr::WideRec*)Q.wrec.p;
+    B.tile_counts = (uint32_t*)Q.tile_counts.p;
+    B.tiles = (uint32_t*)Q.tiles.p;
+    B.items = (gr::RasterItem*)Q.items.p;
+    B.max_items = max_items;
+    B.slabs = (unsigned long long*)Q.slabs.p;
+    B.max_slabs = max_slabs;
+    B.part_refs = part_refs;
+    B.wgcount = (uint32_t*)Q.wgcount.p;
+    B.refs = (uint32_t*)Q.refs.p;
+    B.refs_cap = refs_cap;
+    B.ctrl = (gr::DevControl*)Q.ctrl.p;
+    B.okey = (uint32_t*)Q.okey.p;
+    B.orec = (gr::OverlayRec*)Q.orec.p;
+    B.textures = (const gr::DevTexture*)c->d_textures.p;
+    return GR_OK;
+}
+
+// Texture descriptor table, rebuilt after textures changed (read by the raster
+// of a batch that may still be running on the other pipe: wait for it first).
+gr_status refresh_textures(gr_context* c, hipStream_t s) {
+    if (!c->tex_dirty) return GR_OK;
+    std::vector<gr::DevTexture> tt(std::max<size_t>(1, c->textures.size()));
+    for (size_t i = 0; i < c->textures.size(); i++) tt[i] = c->textures[i].dt;
+    gr_status st;
+    if ((st = sync_all(c))) return st;
+    if ((st = ensure(c, c->d_textures, tt.size() * sizeof(gr::DevTexture)))) return st;
+    if ((st = upload(c, c->d_textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture), s)))
+        return st;
+    c->tex_dirty = false;
+    return GR_OK;
+}
+
+unsigned long long refs_capacity_of(const gr_context* c) {
+    return (unsigned long long)std::min<int64_t>(std::max<int64_t>(c->cfg.refs_capacity, 1024),
+                                                 kMaxRefs);
+}
+
+// The whole pipeline for a batch of frames on the next pipe; outputs are
+// device pointers.
+gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_object* objs,
+                  int no) {
+    const int TW = c->cfg.tile_w, TH = c->cfg.tile_h;
+    std::vector<gr::DevFrame> frames(std::max(nf, 1));
+    std::vector<gr::DevObject> dobj(std::max(no, 1));
+    std::vector<int> xcd(xcd_ints(std::max(nf, 1)));
+    Plan P;
+    gr_status st = make_plan(c, bf, nf, objs, no, TW, TH, frames.data(), dobj.data(), xcd.data(), P);
+    if (st) return st;
 
     // The pipe of this batch: the other one than the last batch's when the
     // context runs on its own streams with two pipes, else pipe 0 on c->stream.
     const bool two = c->stream == c->own && c->cfg.pipelines >= 2 && !c->one_pipe;
     const int pi = two ? (c->last_pipe + 1) & 1 : 0;
-    Pipe& P = c->pipes[pi];
-    P.stream = pi == 0 ? c->stream : c->own1;
-    const hipStream_t s = P.stream;
-    gr_status st;
+    Pipe& Q = c->pipes[pi];
+    Q.stream = pi == 0 ? c->stream : c->own1;
+    const hipStream_t s = Q.stream;
 
-    // Texture descriptor table (read by the raster of a batch that may still be
-    // running on the other pipe: wait for it before replacing the table).
-    if (c->tex_dirty) {
-        std::vector<gr::DevTexture> tt(std::max<size_t>(1, c->textures.size()));
-        for (size_t i = 0; i < c->textures.size(); i++) tt[i] = c->textures[i].dt;
-        if ((st = sync_all(c))) return st;
-        if ((st = ensure(c, c->d_textures, tt.size() * sizeof(gr::DevTexture)))) return st;
-        if ((st = upload(c, c->d_textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture), s)))
-            return st;
-        c->tex_dirty = false;
-    }
-
-    const unsigned long long refs_cap =
-        (unsigned long long)std::max<int64_t>(c->cfg.refs_capacity, 1 << 16);
-    const size_t off_objs = align256((size_t)nf * sizeof(gr::DevFrame));
-    const size_t off_stats = off_objs + align256((size_t)nobj * sizeof(gr::DevObject));
-    const size_t off_counters = off_stats + align256((size_t)nf * gr::kNumStats * 8);
-    const size_t off_over = off_counters + align256(sizeof(gr::DevBatchCounters));
-    const size_t off_xcd = off_over + align256((size_t)nf * 4);
-    const size_t xcd_ints = 8 * ((size_t)nf + 1);
-    const size_t batch_bytes = off_xcd + align256(xcd_ints * sizeof(int));
-    std::vector<int> xcd(xcd_ints), nface(nf);
-    for (int f = 0; f < nf; f++) nface[f] = frames[f].face_blocks;
-    int face_grid = 0;
-    gr::xcd_tables(nface.data(), nf, xcd.data(), &face_grid);
-
-    if ((st = ensure(c, P.batch, batch_bytes))) return st;
-    if ((st = ensure(c, P.rrec, slots * sizeof(gr::RasterRec)))) return st;
-    if ((st = ensure(c, P.srec, slots * sizeof(gr::ShadeRec)))) return st;
-    if ((st = ensure(c, P.wrec, slots * sizeof(gr::WideRec)))) return st;
-    // Tile counts are zero between draws (k_raster re-zeroes the ones it used);
-    // a new allocation starts zeroed.
-    if (P.tile_counts.cap < (size_t)tiles * 4) {
-        void* old = P.tile_counts.p;
-        if ((st = ensure(c, P.tile_counts, (size_t)tiles * 4))) return st;
-        if (P.tile_counts.p != old)
-            GR_HIP(hipMemsetAsync(P.tile_counts.p, 0, P.tile_counts.cap, s));
-    }
-    if ((st = ensure(c, P.tiles, 3 * (size_t)tiles * sizeof(uint32_t)))) return st;
-    // Split tiles: a tile is split only when its list is longer than part_refs,
-    // so the items of a batch number at most refs_cap / part_refs and the split
-    // tiles at most refs_cap / (part_refs + 1); slabs are further capped by a
-    // memory budget (a tile without a slab is rasterised whole).
-    const uint32_t part_refs = (uint32_t)c->cfg.tile_split;
-    const uint32_t max_items = (uint32_t)std::min<unsigned long long>(refs_cap / part_refs + 1, 0xFFFFFFF0ull);
-    const size_t slab_bytes = (size_t)TW * TH * 8;
-    const uint32_t max_slabs = (uint32_t)std::max<unsigned long long>(
-        1, std::min<unsigned long long>(refs_cap / (part_refs + 1ull) + 1, kSlabBudget / slab_bytes));
-    if ((st = ensure(c, P.items, (size_t)max_items * sizeof(gr::RasterItem)))) return st;
-    if (P.slabs.cap < (size_t)max_slabs * slab_bytes) {
-        void* old = P.slabs.p;
-        if ((st = ensure(c, P.slabs, (size_t)max_slabs * slab_bytes))) return st;
-        if (P.slabs.p != old) GR_HIP(hipMemsetAsync(P.slabs.p, 0, P.slabs.cap, s));
-    }
-    if ((st = ensure(c, P.wgcount, std::max(fblocks, 1) * sizeof(uint32_t)))) return st;
-    if ((st = ensure(c, P.refs, refs_cap * sizeof(uint32_t)))) return st;
-    if (!P.ctrl.p) {
-        if ((st = ensure(c, P.ctrl, sizeof(gr::DevControl)))) return st;
-        GR_HIP(hipMemsetAsync(P.ctrl.p, 0, sizeof(gr::DevControl), s));
-    }
-    // Overlay keys are zero between draws (k_raster re-zeroes what it reads);
-    // a new allocation starts zeroed.
-    if (okey_words && (st = ensure(c, P.orec, slots * sizeof(gr::OverlayRec)))) return st;
-    if (okey_words && P.okey.cap < okey_words * 4) {
-        void* old = P.okey.p;
-        if ((st = ensure(c, P.okey, okey_words * 4))) return st;
-        if (P.okey.p != old) GR_HIP(hipMemsetAsync(P.okey.p, 0, P.okey.cap, s));
-    }
-
-    // Offsets -> device pointers, then one upload of the whole batch header.
-    for (auto& d : frames) {
-        if (d.okey) d.okey = (uint32_t*)((char*)P.okey.p + ((uintptr_t)d.okey - 1));
-        if (d.orec) d.orec = (gr::OverlayRec*)((char*)P.orec.p + ((uintptr_t)d.orec - 1));
-        d.ctrl = (gr::DevControl*)P.ctrl.p;
-        d.rrec = (gr::RasterRec*)((char*)P.rrec.p + (uintptr_t)d.rrec);
-        d.srec = (gr::ShadeRec*)((char*)P.srec.p + (uintptr_t)d.srec);
-        d.wrec = (gr::WideRec*)((char*)P.wrec.p + (uintptr_t)d.wrec);
-    }
+    if ((st = refresh_textures(c, s))) return st;
+    Bufs B;
+    if ((st = size_bufs(c, P, Q, refs_capacity_of(c), (uint32_t)c->cfg.tile_split, TW, TH, s, B)))
+        return st;
     {
         void* h;
         int slot;
-        if ((st = staging_get(c, batch_bytes, &h, &slot))) return st;
-        std::memset(h, 0, batch_bytes);
-        std::memcpy(h, frames.data(), (size_t)nf * sizeof(gr::DevFrame));
-        if (nobj) std::memcpy((char*)h + off_objs, dobj.data(), (size_t)nobj * sizeof(gr::DevObject));
-        std::memcpy((char*)h + off_xcd, xcd.data(), xcd.size() * sizeof(int));
-        GR_HIP(hipMemcpyAsync(P.batch.p, h, batch_bytes, hipMemcpyHostToDevice, s));
+        if ((st = staging_get(c, P.header_bytes, &h, &slot))) return st;
+        write_header(P, B, h);
+        GR_HIP(hipMemcpyAsync(B.header, h, P.header_bytes, hipMemcpyHostToDevice, s));
         if ((st = staging_mark(c, slot, s))) return st;
     }
-    char* bb = (char*)P.batch.p;
-
-    gr::LaunchParams p{};
-    p.objs = (const gr::DevObject*)(bb + off_objs);
-    p.nobj = nobj;
-    p.frames = (const gr::DevFrame*)bb;
-    p.nframes = nf;
-    p.textures = (const gr::DevTexture*)c->d_textures.p;
-    p.face_blocks = fblocks;
-    p.total_tiles = tiles;
-    p.tw_log2 = twl;
-    p.th_log2 = thl;
-    p.tile_counts = (uint32_t*)P.tile_counts.p;
-    p.tile_offsets = (uint32_t*)P.tiles.p;
-    p.tile_cursor = (uint32_t*)P.tiles.p + tiles;
-    p.tile_slab = (uint32_t*)P.tiles.p + 2 * (size_t)tiles;
-    p.items = (gr::RasterItem*)P.items.p;
-    p.max_items = max_items;
-    p.slabs = (unsigned long long*)P.slabs.p;
-    p.max_slabs = max_slabs;
-    p.part_refs = part_refs;
-    p.frame_overflow = (uint32_t*)(bb + off_over);
-    p.wg_count = (uint32_t*)P.wgcount.p;
-    p.refs = (uint32_t*)P.refs.p;
-    p.refs_capacity = refs_cap;
-    p.stats = (unsigned long long*)(bb + off_stats);
-    p.counters = (gr::DevBatchCounters*)(bb + off_counters);
-    p.ctrl = (gr::DevControl*)P.ctrl.p;
-    p.xcd_face = (const int*)(bb + off_xcd);
-    p.face_grid = face_grid;
-
-    timed_launch(c, "k_face", s, [&] { gr::launch_face(p, s); });
-    timed_launch(c, "k_bin_scan", s, [&] { gr::launch_bin_scan(p, s); });
-    timed_launch(c, "k_bin_fill", s, [&] { gr::launch_bin_fill(p, s); });
-    if (okey_words) timed_launch(c, "k_overlay", s, [&] { gr::launch_overlay(p, s); });
-    bool ok = true;
-    timed_launch(c, "k_raster_parts", s, [&] { ok = gr::launch_raster_parts(p, s); });
+    const gr::LaunchParams p = launch_params(P, B);
     // Frames are written in submission order: this raster starts after the
     // previous batch's (on the other pipe).
+    hipEvent_t after = nullptr;
     if (c->last_pipe >= 0 && c->last_pipe != pi && c->pipes[c->last_pipe].raster_pending)
-        GR_HIP(hipStreamWaitEvent(s, c->pipes[c->last_pipe].raster_done, 0));
-    if (ok) timed_launch(c, "k_raster", s, [&] { ok = gr::launch_raster(p, s); });
-    // Drawn over the finished frame, before the event the next batch's raster
-    // waits for (it may write the same buffers).
-    if (ok && decorate) timed_launch(c, "k_decorate", s, [&] { gr::launch_decorate(p, s); });
-    hipError_t le = hipGetLastError();
-    if (!ok || le != hipSuccess) {
-        // Leave the tile counters zeroed for the next draw.
-        (void)hipMemsetAsync(P.tile_counts.p, 0, P.tile_counts.cap, s);
-        if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
-        return fail(GR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(le));
-    }
-    if (!P.raster_done) GR_HIP(hipEventCreateWithFlags(&P.raster_done, hipEventDisableTiming));
-    GR_HIP(hipEventRecord(P.raster_done, s));
-    P.raster_pending = true;
+        after = c->pipes[c->last_pipe].raster_done;
+    if ((st = launch_plan(c, P, p, s, after))) return st;
+    if (!Q.raster_done) GR_HIP(hipEventCreateWithFlags(&Q.raster_done, hipEventDisableTiming));
+    GR_HIP(hipEventRecord(Q.raster_done, s));
+    Q.raster_pending = true;
     c->last_pipe = pi;
     c->last_frames = nf;
     c->last_stats = p.stats;
@@ -630,6 +746,22 @@ void to_gr_stats(const unsigned long long* q, gr_stats* s) {
 }
 
 // Wait for every pipe, then read and clear the sticky control words of both.
+// Reads the sticky control words at d (device) after the work on stream s, and
+// clears them when set.
+gr_status read_ctrl(gr_context* c, void* d, hipStream_t s, gr::DevControl* ctl) {
+    GR_HIP(hipMemcpyAsync(c->h_ctrl, d, sizeof *ctl, hipMemcpyDeviceToHost, s));
+    GR_HIP(hipStreamSynchronize(s));
+    std::memcpy(ctl, c->h_ctrl, sizeof *ctl);
+    if (ctl->unsupported || ctl->overflow) {
+        gr::launch_fill32((uint32_t*)d, 0u, sizeof *ctl / 4, s);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipStreamSynchronize(s));
+    }
+    return GR_OK;
+}
+
+// Wait for every pipe, then read and clear the sticky control words of both
+// and of the no-allocation draw.
 gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
     gr_status st = sync_all(c);
     if (st) return st;
@@ -639,24 +771,27 @@ gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
     for (Pipe& pp : c->pipes) {
         if (!pp.ctrl.p) continue;
         gr::DevControl ctl;
-        GR_HIP(hipMemcpyAsync(c->h_ctrl, pp.ctrl.p, sizeof ctl, hipMemcpyDeviceToHost, pp.stream));
-        GR_HIP(hipStreamSynchronize(pp.stream));
-        std::memcpy(&ctl, c->h_ctrl, sizeof ctl);
-        if (ctl.unsupported || ctl.overflow) {
-            unsupported = unsupported || ctl.unsupported;
-            overflow = overflow || ctl.overflow;
-            *needed = std::max(*needed, ctl.refs_needed);
-            GR_HIP(hipMemsetAsync(pp.ctrl.p, 0, sizeof ctl, pp.stream));
-            GR_HIP(hipStreamSynchronize(pp.stream));
-        }
+        if ((st = read_ctrl(c, pp.ctrl.p, pp.stream, &ctl))) return st;
+        unsupported = unsupported || ctl.unsupported;
+        overflow = overflow || ctl.overflow;
+        if (ctl.overflow) *needed = std::max(*needed, ctl.refs_needed);
+    }
+    bool noalloc_overflow = false;
+    if (c->noalloc_ctrl) {
+        gr::DevControl ctl;
+        if ((st = read_ctrl(c, c->noalloc_ctrl, c->stream, &ctl))) return st;
+        noalloc_overflow = ctl.overflow != 0;
     }
     if (unsupported) return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+    if (noalloc_overflow && !overflow)
+        return fail(GR_ERR_CAPACITY,
+                    "tile lists of a no-allocation draw exceeded its fixed scratch array");
     if (overflow) return GR_ERR_CAPACITY;
     return GR_OK;
 }
 
 void grow_refs(gr_context* c, unsigned long long needed) {
-    int64_t want = (int64_t)needed + (int64_t)needed / 4 + 1024;
+    int64_t want = std::min<int64_t>((int64_t)needed + (int64_t)needed / 4 + 1024, kMaxRefs);
     if (want > c->cfg.refs_capacity) c->cfg.refs_capacity = want;
 }
 
@@ -677,6 +812,16 @@ void free_mesh(MeshRes& mr) {
 }
 
 bool is_pow2(int32_t x) { return x != 0 && (x & (x - 1)) == 0; }
+
+void free_pipe_bufs(Pipe& pp) {
+    DevBuf* pb[] = {&pp.batch, &pp.rrec,    &pp.srec, &pp.wrec,  &pp.tile_counts,
+                    &pp.tiles, &pp.wgcount, &pp.refs, &pp.ctrl,  &pp.items,
+                    &pp.slabs, &pp.okey,    &pp.orec};
+    for (DevBuf* b : pb) {
+        if (b->p) (void)hipFree(b->p);
+        *b = DevBuf{};
+    }
+}
 
 }  // namespace
 
@@ -727,6 +872,109 @@ gr_status check_request(gr_context* c, const gr_frame* frame, const gr_object* o
     return check_frame(c, *frame, objects, num_objects);
 }
 
+// The no-allocation draw. Layout of [base, base + bytes): control words (sticky
+// until gr_synchronize reads them), batch header, texture table, records, tile
+// counters, tile offsets/cursors/slabs, face-block counts, then tile-list
+// entries in what remains. Tiles are never split (no slabs or work items).
+// Host data reaches the array through kernel arguments (k_put) and zeroing is
+// a fill kernel, so nothing here allocates memory of any kind, and the launch
+// sequence holds no memset or host-buffer copy.
+gr_status enqueue_noalloc(gr_context* c, const gr_frame* frame, const gr_object* objs, int32_t no,
+                          void* d_rgba, void* d_depth, void* base, size_t bytes) {
+    try {
+        GR_HIP(hipSetDevice(c->device));
+        if (no > kNoAllocMaxObjects)
+            return ::fail(GR_ERR_INVALID_ARGUMENT, "the no-allocation draw takes at most 16 objects");
+        if (frame->flags & (GR_SHOW_EDGES | GR_SHOW_VERTICES))
+            return ::fail(GR_ERR_UNSUPPORTED,
+                          "edge and vertex overlays need per-pixel keys: use gr_draw_device");
+        const int TW = c->cfg.tile_w, TH = c->cfg.tile_h;
+        gr_batch_frame bf;
+        bf.frame = *frame;
+        bf.first_object = 0;
+        bf.num_objects = no;
+        bf.d_rgba = d_rgba;
+        bf.d_depth = d_depth;
+        gr::DevFrame frames[1];
+        gr::DevObject dobj[kNoAllocMaxObjects];
+        int xcd[16];
+        Plan P;
+        gr_status st = make_plan(c, &bf, 1, objs, no, TW, TH, frames, dobj, xcd, P);
+        if (st) return st;
+
+        const size_t ntex = std::max<size_t>(1, c->textures.size());
+        size_t off = 0;
+        auto take = [&](size_t n) {
+            const size_t o = off;
+            off += align256(std::max<size_t>(n, 4));
+            return (char*)base + o;
+        };
+        Bufs B;
+        B.ctrl = (gr::DevControl*)take(sizeof(gr::DevControl));
+        B.header = take(P.header_bytes);
+        B.textures = (const gr::DevTexture*)take(ntex * sizeof(gr::DevTexture));
+        B.rrec = (gr::RasterRec*)take(P.slots * sizeof(gr::RasterRec));
+        B.srec = (gr::ShadeRec*)take(P.slots * sizeof(gr::ShadeRec));
+        B.wrec = (gr::WideRec*)take(P.slots * sizeof(gr::WideRec));
+        B.tile_counts = (uint32_t*)take((size_t)P.tiles * 4);
+        B.tiles = (uint32_t*)take(3 * (size_t)P.tiles * 4);
+        B.wgcount = (uint32_t*)take((size_t)std::max(P.fblocks, 1) * 4);
+        const size_t min_refs = 1u << 16;
+        if (off + min_refs * 4 > bytes)
+            return ::fail(GR_ERR_INVALID_ARGUMENT,
+                          "frame too large for the no-allocation draw: needs " +
+                              std::to_string((off + min_refs * 4) >> 20) +
+                              " MiB of its 32 MiB device array (use gr_draw_device)");
+        B.refs = (uint32_t*)((char*)base + off);
+        B.refs_cap = std::min<unsigned long long>((bytes - off) / 4, (unsigned long long)kMaxRefs);
+        B.part_refs = 0xFFFFFFFFu;  // no split tiles
+        B.items = nullptr;
+        B.max_items = 0;
+        B.slabs = nullptr;
+        B.max_slabs = 0;
+
+        const hipStream_t s = c->stream;
+        // The previous draw of the context may run on the other pipe: this
+        // one's buffers are its own, but its frame is written after that one's.
+        hipEvent_t after = nullptr;
+        if (c->last_pipe == 1 && c->pipes[1].raster_pending && c->pipes[1].stream != s)
+            after = c->pipes[1].raster_done;
+        // Header and texture table through kernel arguments, from the stack.
+        unsigned char hbuf[8192];
+        if (P.header_bytes > sizeof hbuf)
+            return ::fail(GR_ERR_INVALID_ARGUMENT, "batch header too large");
+        write_header(P, B, hbuf);
+        gr::launch_put(B.header, hbuf, P.header_bytes, s);
+        {
+            constexpr size_t kChunk = (gr::kPutWords * 4) / sizeof(gr::DevTexture);
+            gr::DevTexture tt[kChunk];
+            for (size_t t0 = 0; t0 < ntex; t0 += kChunk) {
+                const size_t n = std::min(kChunk, ntex - t0);
+                for (size_t i = 0; i < n; i++)
+                    tt[i] = t0 + i < c->textures.size() ? c->textures[t0 + i].dt : gr::DevTexture{};
+                gr::launch_put((char*)B.textures + t0 * sizeof(gr::DevTexture), tt,
+                               n * sizeof(gr::DevTexture), s);
+            }
+        }
+        // Tile counters start at zero (a failed earlier draw may have left some).
+        gr::launch_fill32(B.tile_counts, 0u, (size_t)P.tiles, s);
+        GR_HIP(hipGetLastError());
+        const gr::LaunchParams p = launch_params(P, B);
+        if ((st = launch_plan(c, P, p, s, after))) return st;
+        Pipe& Q = c->pipes[0];
+        Q.stream = s;
+        GR_HIP(hipEventRecord(Q.raster_done, s));
+        Q.raster_pending = true;
+        c->last_pipe = 0;
+        c->last_frames = 1;
+        c->last_stats = p.stats;
+        c->noalloc_ctrl = B.ctrl;
+        return GR_OK;
+    } catch (...) {
+        return ::fail(GR_ERR_DEVICE, "unexpected exception");
+    }
+}
+
 }  // namespace gr_internal
 
 extern "C" {
@@ -760,6 +1008,16 @@ gr_status gr_create(int32_t device, gr_context** out) {
             delete c;
             return fail(GR_ERR_OUT_OF_MEMORY, "pinned allocation failed");
         }
+        // Set-up of the no-allocation draw: its array's address on this device,
+        // and the event its raster records (created here, not in the draw).
+        gr_status sst = gr_internal::static_scratch_init(device);
+        if (!sst && hipEventCreateWithFlags(&c->pipes[0].raster_done, hipEventDisableTiming) !=
+                        hipSuccess)
+            sst = fail(GR_ERR_DEVICE, "hipEventCreate failed");
+        if (sst) {
+            gr_destroy(c);
+            return sst;
+        }
         *out = c;
         return GR_OK;
     } catch (const std::bad_alloc&) {
@@ -773,6 +1031,7 @@ void gr_destroy(gr_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     for (auto& m : c->meshes) free_mesh(m);
     for (auto& t : c->textures)
         if (t.texels) (void)hipFree(t.texels);
@@ -781,11 +1040,7 @@ void gr_destroy(gr_context* c) {
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (Pipe& pp : c->pipes) {
-        DevBuf* pb[] = {&pp.batch, &pp.rrec,    &pp.srec,  &pp.wrec,  &pp.tile_counts,
-                        &pp.tiles, &pp.wgcount, &pp.refs,  &pp.ctrl,  &pp.items,
-                        &pp.slabs, &pp.okey,    &pp.orec};
-        for (DevBuf* b : pb)
-            if (b->p) (void)hipFree(b->p);
+        free_pipe_bufs(pp);
         if (pp.raster_done) (void)hipEventDestroy(pp.raster_done);
     }
     if (c->own1) (void)hipStreamDestroy(c->own1);
@@ -817,7 +1072,8 @@ gr_status gr_set_config(gr_context* c, const gr_config* cfg) {
     if (n.raster_waves != 4) return fail(GR_ERR_INVALID_ARGUMENT, "raster_waves must be 4");
     if (n.pipelines != 1 && n.pipelines != 2)
         return fail(GR_ERR_INVALID_ARGUMENT, "pipelines must be 1 or 2");
-    if (n.refs_capacity < 1024) return fail(GR_ERR_INVALID_ARGUMENT, "refs_capacity too small");
+    if (n.refs_capacity < 1024 || n.refs_capacity > kMaxRefs)
+        return fail(GR_ERR_INVALID_ARGUMENT, "refs_capacity must be in [1024, 2^32 - 1]");
     if (n.tile_split < 64 || n.tile_split > (1 << 24))
         return fail(GR_ERR_INVALID_ARGUMENT, "tile_split must be in [64, 2^24]");
     c->cfg = n;
@@ -1149,6 +1405,7 @@ gr_status gr_synchronize(gr_context* c) {
     unsigned long long needed = 0;
     gr_status st = sync_and_check(c, &needed);
     if (st == GR_ERR_CAPACITY) {
+        if (!needed) return st;  // the no-allocation draw's fixed array: message set
         grow_refs(c, needed);
         return fail(GR_ERR_CAPACITY, "tile-list capacity exceeded; redraw the batch");
     }

@@ -36,4 +36,13 @@ gr_status check_request(gr_context* c, const gr_frame* frame, const gr_object* o
 // Keep every batch on pipe 0 (the serving loop overlaps its own readback).
 void set_one_pipe(gr_context* c, bool on);
 
+// The no-allocation draw's fixed device array (gr_static.hip): resolve its
+// address on `device` (current), and look it up.
+gr_status static_scratch_init(int device);
+void* static_scratch(int device, size_t* bytes);
+// One frame drawn with every per-draw buffer carved from [base, base + bytes):
+// no allocation of any kind on the way.
+gr_status enqueue_noalloc(gr_context* c, const gr_frame* frame, const gr_object* objs, int32_t no,
+                          void* d_rgba, void* d_depth, void* base, size_t bytes);
+
 }  // namespace gr_internal

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstring>
 
 #include "gr_device.h"
 #include "gr_launch.h"
@@ -2125,6 +2126,40 @@ void launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_copy8, dim3((unsigned)blocks), dim3(256), 0, s,
                            (const uint8_t*)src, (uint8_t*)dst, n);
+}
+
+// Fills n 32-bit words with v: the zeroing of tile counters, slabs and control
+// words (a kernel rather than hipMemsetAsync, so that the launch sequence of a
+// batch can be captured in a graph).
+__global__ __launch_bounds__(256) void k_fill32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = v;
+}
+
+void launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+}
+
+// Writes c.n words (at most kPutWords) carried by the kernel's arguments to
+// dst: host data reaches device memory without a host buffer that must outlive
+// the call, and without an allocation.
+__global__ __launch_bounds__(256) void k_put(uint32_t* __restrict__ dst, PutChunk c) {
+    for (uint32_t i = threadIdx.x; i < c.n; i += 256) dst[i] = c.w[i];
+}
+
+void launch_put(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const size_t words = (bytes + 3) / 4;
+    for (size_t w0 = 0; w0 < words; w0 += kPutWords) {
+        PutChunk c;
+        c.n = (uint32_t)std::min<size_t>(kPutWords, words - w0);
+        c.pad = 0;
+        const size_t b0 = w0 * 4, nb = std::min<size_t>((size_t)c.n * 4, bytes - b0);
+        std::memset(c.w, 0, sizeof c.w);
+        std::memcpy(c.w, (const char*)src + b0, nb);
+        hipLaunchKernelGGL(k_put, dim3(1), dim3(256), 0, s, (uint32_t*)dst + w0, c);
+    }
 }
 
 template <int TW, int TH>

@@ -42,6 +42,13 @@ struct LaunchParams {
     int face_grid;
 };
 
+// Host words carried to device memory by one k_put launch (its argument).
+constexpr int kPutWords = 496;
+struct PutChunk {
+    uint32_t n, pad;
+    uint32_t w[kPutWords];
+};
+
 // Prefix tables of the XCD-aware order for per-frame unit counts `n`.
 void xcd_tables(const int* n, int nframes, int* table, int* grid);
 
@@ -58,5 +65,8 @@ void launch_fog(void* rgba, const void* depth, uint64_t n, float fog_start, floa
 void launch_bin_fill(const LaunchParams& p, hipStream_t s);
 bool launch_raster_parts(const LaunchParams& p, hipStream_t s);
 bool launch_raster(const LaunchParams& p, hipStream_t s);
+void launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// Copies `bytes` host bytes to dst (4-byte aligned) through kernel arguments.
+void launch_put(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 }  // namespace gr

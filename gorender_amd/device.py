@@ -161,6 +161,15 @@ class Context:
         _lib.check(self.lib.gr_draw_device(self.handle, ctypes.byref(fr), objs,
                                            len(s.objects), d_rgba, d_depth))
 
+    def draw_device_noalloc(self, s: SceneData, mesh_handles: Sequence[int], d_rgba: int,
+                            d_depth: int) -> None:
+        """gr_draw_device_noalloc: the same draw with every per-draw buffer in the
+        library's fixed device array (no allocation)."""
+        fr = frame_struct(s.frame)
+        objs = self.objects_struct(s, mesh_handles)
+        _lib.check(self.lib.gr_draw_device_noalloc(self.handle, ctypes.byref(fr), objs,
+                                                   len(s.objects), d_rgba, d_depth))
+
     def prepare_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
                       outputs: Sequence[Tuple[int, int]]):
         """Build the ctypes arguments of a gr_draw_batch call once (for loops
@@ -266,3 +275,4 @@ class DeviceBuffer:
             self.free()
         except Exception:
             pass
+

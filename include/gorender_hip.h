@@ -225,6 +225,22 @@ gr_status gr_draw_device(gr_context *ctx, const gr_frame *frame, const gr_object
 /* Many independent frames in one set of launches (batched form). */
 gr_status gr_draw_batch(gr_context *ctx, const gr_batch_frame *frames, int32_t num_frames,
                         const gr_object *objects, int32_t num_objects);
+/* gr_draw_device that allocates nothing: no counterpart in the reference, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Every per-draw buffer
+ * lives in one fixed 32 MiB device array per GPU, held until the process ends;
+ * host data reaches it through kernel arguments, and the draw calls no
+ * allocation, free or memset. Limits: one frame of at most 16 objects, no edge
+ * or vertex overlays (GR_ERR_UNSUPPORTED), and records + tile counters for the
+ * frame plus 2^16 tile-list entries within 32 MiB (about 25k faces at 1080p):
+ * larger frames -> GR_ERR_INVALID_ARGUMENT. Tile lists longer than what is
+ * left of the array -> GR_ERR_CAPACITY from gr_synchronize. The array is
+ * shared by every call on the GPU and keeps the last call's contents: calls
+ * that overlap on one GPU (two contexts or streams at once) race; calls on
+ * one context's stream are ordered. The image equals gr_draw_device's. */
+gr_status gr_draw_device_noalloc(gr_context *ctx, const gr_frame *frame,
+                                 const gr_object *objects, int32_t num_objects, void *d_rgba,
+                                 void *d_depth);
 /* Wait for queued draws; returns GR_ERR_CAPACITY if one of them overflowed its
  * bins (its output is then incomplete and must be drawn again). */
 gr_status gr_synchronize(gr_context *ctx);
