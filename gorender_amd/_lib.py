@@ -87,6 +87,12 @@ class gr_config(Structure):
                 ("reserved", c_int32)]
 
 
+class gr_tune_result(Structure):
+    _fields_ = [("tile_w", c_int32), ("tile_h", c_int32), ("tile_split", c_int32),
+                ("raster_waves", c_int32), ("ms_per_launch", c_double),
+                ("checksum", ctypes.c_uint64)]
+
+
 class gr_server_config(Structure):
     _fields_ = [("target_budget", c_int64), ("max_batch", c_int32), ("max_inflight", c_int32),
                 ("colour_only", c_int32), ("reserved", c_int32)]
@@ -131,6 +137,13 @@ SIGNATURES = {
     "gr_draw_batch": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32]),
     "gr_draw_device_noalloc": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32,
                                          c_void_p, c_void_p]),
+    "gr_batch_graph_create": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32,
+                                        POINTER(c_void_p)]),
+    "gr_batch_graph_update": (c_int32, [c_void_p, c_void_p, c_int32]),
+    "gr_batch_graph_launch": (c_int32, [c_void_p]),
+    "gr_batch_graph_destroy": (None, [c_void_p]),
+    "gr_autotune": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32,
+                              c_void_p, c_int32, POINTER(c_int32), POINTER(gr_config)]),
     "gr_synchronize": (c_int32, [c_void_p]),
     "gr_last_stats": (c_int32, [c_void_p, c_void_p, c_int32, POINTER(c_int32)]),
     "gr_device_alloc": (c_int32, [c_void_p, ctypes.c_size_t, POINTER(c_void_p)]),
@@ -154,6 +167,7 @@ SIGNATURES = {
     "gr_texture_load": (c_int32, [c_void_p, ctypes.c_char_p, POINTER(c_int32)]),
     "gr_check_exact_division": (c_int32, [c_void_p, ctypes.c_uint64, ctypes.c_uint64,
                                           POINTER(ctypes.c_uint64), POINTER(ctypes.c_float)]),
+    "gr_device_checksum": (c_int32, [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
     "gr_copy_to_host": (c_int32, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t]),
     "gr_set_kernel_timing": (c_int32, [c_void_p, c_int32]),
     "gr_kernel_times": (c_int32, [c_void_p, POINTER(c_char_p), POINTER(c_double),

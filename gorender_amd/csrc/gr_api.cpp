@@ -111,6 +111,8 @@ struct gr_context {
     // Control words of the no-allocation draw (in its fixed device array),
     // once this context has used it.
     gr::DevControl* noalloc_ctrl = nullptr;
+    // Live batch graphs (their control words are checked by gr_synchronize).
+    std::vector<gr_batch_graph*> graphs;
 
     gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 0};
 
@@ -760,8 +762,11 @@ gr_status read_ctrl(gr_context* c, void* d, hipStream_t s, gr::DevControl* ctl) 
     return GR_OK;
 }
 
-// Wait for every pipe, then read and clear the sticky control words of both
-// and of the no-allocation draw.
+gr_status graphs_check(gr_context* c, bool* unsupported, bool* overflow,
+                       unsigned long long* needed);
+
+// Wait for every pipe, then read and clear the sticky control words of both,
+// of the batch graphs and of the no-allocation draw.
 gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
     gr_status st = sync_all(c);
     if (st) return st;
@@ -776,6 +781,7 @@ gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
         overflow = overflow || ctl.overflow;
         if (ctl.overflow) *needed = std::max(*needed, ctl.refs_needed);
     }
+    if ((st = graphs_check(c, &unsupported, &overflow, needed))) return st;
     bool noalloc_overflow = false;
     if (c->noalloc_ctrl) {
         gr::DevControl ctl;
@@ -821,6 +827,174 @@ void free_pipe_bufs(Pipe& pp) {
         if (b->p) (void)hipFree(b->p);
         *b = DevBuf{};
     }
+}
+
+}  // namespace
+
+// One batch's launch sequence captured in two HIP graphs, with its own device
+// buffers: counter zeroing, k_face, k_bin_scan, k_bin_fill, k_overlay and
+// k_raster_parts in the geometry graph; k_raster and k_decorate in the raster
+// graph, so that a replay's raster can wait for the previous batch's between
+// the two graph launches while its geometry overlaps that raster.
+struct gr_batch_graph {
+    gr_context* c = nullptr;
+    int TW = 32, TH = 32;
+    uint32_t part_refs = 512;
+    unsigned long long refs_cap = 0;
+    std::vector<gr_batch_frame> bf;  // as created (outputs, object ranges)
+    std::vector<gr_object> objs;
+    std::vector<gr::DevFrame> frames;
+    std::vector<gr::DevObject> dobj;
+    std::vector<int> xcd;
+    Plan P;
+    Pipe buf;          // the graph's buffers (its stream field is unused)
+    DevBuf textures;   // texture table as at creation
+    Bufs B;
+    gr::LaunchParams p{};
+    hipStream_t cap = nullptr;  // capture (and first-run) stream
+    hipGraph_t geo = nullptr, ras = nullptr;
+    hipGraphExec_t geo_x = nullptr, ras_x = nullptr;
+    hipEvent_t done = nullptr;   // after the last replay's raster
+    hipStream_t last = nullptr;  // stream of the last replay
+    bool regrow = false;         // a replay overflowed its tile lists
+    unsigned long long needed = 0;
+};
+
+namespace {
+
+void graph_drop_exec(gr_batch_graph* g) {
+    if (g->geo_x) (void)hipGraphExecDestroy(g->geo_x);
+    if (g->ras_x) (void)hipGraphExecDestroy(g->ras_x);
+    if (g->geo) (void)hipGraphDestroy(g->geo);
+    if (g->ras) (void)hipGraphDestroy(g->ras);
+    g->geo_x = g->ras_x = nullptr;
+    g->geo = g->ras = nullptr;
+}
+
+// Capture of one stage on the graph's capture stream; the capture is always
+// ended, so a failed launch leaves the stream usable.
+template <typename F>
+gr_status capture(gr_batch_graph* g, hipGraph_t* out, hipGraphExec_t* exec, F launches) {
+    GR_HIP(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+    const bool ok = launches();
+    const hipError_t le = hipGetLastError();
+    hipGraph_t gr = nullptr;
+    const hipError_t ee = hipStreamEndCapture(g->cap, &gr);
+    if (!ok || le != hipSuccess || ee != hipSuccess || !gr) {
+        if (gr) (void)hipGraphDestroy(gr);
+        if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
+        return fail(GR_ERR_DEVICE, std::string("graph capture: ") +
+                                       hipGetErrorString(le != hipSuccess ? le : ee));
+    }
+    *out = gr;
+    GR_HIP(hipGraphInstantiate(exec, gr, nullptr, nullptr, 0));
+    return GR_OK;
+}
+
+// (Re)builds the graph's buffers and header at its capacity and captures its
+// two stages.
+gr_status graph_build(gr_batch_graph* g) {
+    gr_context* c = g->c;
+    gr_status st;
+    graph_drop_exec(g);
+    if ((st = size_bufs(c, g->P, g->buf, g->refs_cap, g->part_refs, g->TW, g->TH, g->cap, g->B)))
+        return st;
+    g->B.textures = (const gr::DevTexture*)g->textures.p;
+    std::vector<unsigned char> h(g->P.header_bytes);
+    write_header(g->P, g->B, h.data());
+    GR_HIP(hipMemcpyAsync(g->B.header, h.data(), h.size(), hipMemcpyHostToDevice, g->cap));
+    GR_HIP(hipStreamSynchronize(g->cap));
+    g->p = launch_params(g->P, g->B);
+    const gr::LaunchParams& p = g->p;
+    const Plan& P = g->P;
+    const hipStream_t s = g->cap;
+    // The per-replay counters (stats, batch counters, frame overflow flags)
+    // start at zero; the frames, objects and XCD tables stay as uploaded.
+    char* zero0 = g->B.header + P.off_stats;
+    const size_t zero_words = (P.off_xcd - P.off_stats) / 4;
+    if ((st = capture(g, &g->geo, &g->geo_x, [&] {
+             gr::launch_fill32((uint32_t*)zero0, 0u, zero_words, s);
+             gr::launch_face(p, s);
+             gr::launch_bin_scan(p, s);
+             gr::launch_bin_fill(p, s);
+             if (P.okey_words) gr::launch_overlay(p, s);
+             return gr::launch_raster_parts(p, s);
+         })))
+        return st;
+    if ((st = capture(g, &g->ras, &g->ras_x, [&] {
+             if (!gr::launch_raster(p, s)) return false;
+             if (P.decorate) gr::launch_decorate(p, s);
+             return true;
+         })))
+        return st;
+    return GR_OK;
+}
+
+// Reads and clears the control words of every batch graph (after sync_all); a
+// graph whose replay overflowed regrows at its next launch.
+gr_status graphs_check(gr_context* c, bool* unsupported, bool* overflow,
+                       unsigned long long* needed) {
+    for (gr_batch_graph* g : c->graphs) {
+        if (!g->B.ctrl) continue;
+        gr::DevControl ctl;
+        gr_status st = read_ctrl(c, g->B.ctrl, g->cap, &ctl);
+        if (st) return st;
+        *unsupported = *unsupported || ctl.unsupported;
+        if (ctl.overflow) {
+            *overflow = true;
+            *needed = std::max(*needed, ctl.refs_needed);
+            g->regrow = true;
+            g->needed = std::max(g->needed, ctl.refs_needed);
+        }
+    }
+    return GR_OK;
+}
+
+unsigned long long grown(unsigned long long needed) {
+    return (unsigned long long)std::min<int64_t>((int64_t)needed + (int64_t)needed / 4 + 1024,
+                                                 kMaxRefs);
+}
+
+gr_status graph_launch(gr_batch_graph* g) {
+    gr_context* c = g->c;
+    gr_status st;
+    if (g->regrow) {
+        if ((st = sync_all(c))) return st;
+        g->refs_cap = std::max(g->refs_cap, grown(g->needed));
+        if ((st = graph_build(g))) return st;
+        g->regrow = false;
+        g->needed = 0;
+    }
+    const bool two = c->stream == c->own && c->cfg.pipelines >= 2 && !c->one_pipe;
+    const int pi = two ? (c->last_pipe + 1) & 1 : 0;
+    Pipe& Q = c->pipes[pi];
+    Q.stream = pi == 0 ? c->stream : c->own1;
+    const hipStream_t s = Q.stream;
+    // The previous replay of this graph (its buffers) and the previous batch's
+    // raster (frame order) come first.
+    if (g->last && g->last != s) GR_HIP(hipStreamWaitEvent(s, g->done, 0));
+    GR_HIP(hipGraphLaunch(g->geo_x, s));
+    if (c->last_pipe >= 0 && c->last_pipe != pi && c->pipes[c->last_pipe].raster_pending)
+        GR_HIP(hipStreamWaitEvent(s, c->pipes[c->last_pipe].raster_done, 0));
+    GR_HIP(hipGraphLaunch(g->ras_x, s));
+    if (!Q.raster_done) GR_HIP(hipEventCreateWithFlags(&Q.raster_done, hipEventDisableTiming));
+    GR_HIP(hipEventRecord(Q.raster_done, s));
+    GR_HIP(hipEventRecord(g->done, s));
+    Q.raster_pending = true;
+    g->last = s;
+    c->last_pipe = pi;
+    c->last_frames = g->P.nf;
+    c->last_stats = g->p.stats;
+    return GR_OK;
+}
+
+void graph_free(gr_batch_graph* g) {
+    graph_drop_exec(g);
+    free_pipe_bufs(g->buf);
+    if (g->textures.p) (void)hipFree(g->textures.p);
+    if (g->done) (void)hipEventDestroy(g->done);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    delete g;
 }
 
 }  // namespace
@@ -1032,6 +1206,8 @@ void gr_destroy(gr_context* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)sync_all(c);
+    for (gr_batch_graph* g : c->graphs) graph_free(g);
+    c->graphs.clear();
     for (auto& m : c->meshes) free_mesh(m);
     for (auto& t : c->textures)
         if (t.texels) (void)hipFree(t.texels);
@@ -1540,6 +1716,17 @@ gr_status gr_check_exact_division(gr_context* c, uint64_t count, uint64_t seed,
     return GR_OK;
 }
 
+gr_status gr_device_checksum(gr_context* c, const void* p, size_t bytes, void* digest) {
+    if (!c || !digest || (!p && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    if (bytes % 4) return fail(GR_ERR_INVALID_ARGUMENT, "checksum over whole 32-bit words only");
+    GR_HIP(hipSetDevice(c->device));
+    gr_status st = after_draws(c);
+    if (st) return st;
+    gr::launch_checksum(p, bytes / 4, (unsigned long long*)digest, c->stream);
+    GR_HIP(hipGetLastError());
+    return before_draws(c);
+}
+
 gr_status gr_copy_to_host(gr_context* c, void* dst, const void* src, size_t bytes) {
     if (!c || ((!dst || !src) && bytes)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
     gr_status st = sync_all(c);
@@ -1582,6 +1769,244 @@ void gr_reset_kernel_times(gr_context* c) {
     (void)sync_all(c);
     collect_times(c);
     c->kernel_ms.clear();
+}
+
+gr_status gr_batch_graph_create(gr_context* c, const gr_batch_frame* frames, int32_t nf,
+                                const gr_object* objs, int32_t no, gr_batch_graph** out) {
+    if (!c || !frames || !out || (no > 0 && !objs))
+        return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    gr_batch_graph* g = nullptr;
+    try {
+        GR_HIP(hipSetDevice(c->device));
+        gr_status st = sync_all(c);
+        if (st) return st;
+        g = new gr_batch_graph();
+        g->c = c;
+        g->TW = c->cfg.tile_w;
+        g->TH = c->cfg.tile_h;
+        g->part_refs = (uint32_t)c->cfg.tile_split;
+        g->refs_cap = refs_capacity_of(c);
+        g->bf.assign(frames, frames + std::max(nf, 0));
+        g->objs.assign(objs, objs + std::max(no, 0));
+        g->frames.resize(std::max(nf, 1));
+        g->dobj.resize(std::max(no, 1));
+        g->xcd.resize(xcd_ints(std::max(nf, 1)));
+        st = make_plan(c, frames, nf, objs, no, g->TW, g->TH, g->frames.data(), g->dobj.data(),
+                       g->xcd.data(), g->P);
+        if (!st && (hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&g->done, hipEventDisableTiming) != hipSuccess))
+            st = fail(GR_ERR_DEVICE, "graph stream or event");
+        if (!st) {
+            std::vector<gr::DevTexture> tt(std::max<size_t>(1, c->textures.size()));
+            for (size_t i = 0; i < c->textures.size(); i++) tt[i] = c->textures[i].dt;
+            st = ensure(c, g->textures, tt.size() * sizeof(gr::DevTexture));
+            if (!st && hipMemcpy(g->textures.p, tt.data(), tt.size() * sizeof(gr::DevTexture),
+                                 hipMemcpyHostToDevice) != hipSuccess)
+                st = fail(GR_ERR_DEVICE, "graph texture table");
+        }
+        // Capture, then a first replay on the capture stream that sizes the
+        // tile lists (grown and captured again while they overflow).
+        for (int attempt = 0; !st; attempt++) {
+            if ((st = graph_build(g))) break;
+            if (hipGraphLaunch(g->geo_x, g->cap) != hipSuccess ||
+                hipGraphLaunch(g->ras_x, g->cap) != hipSuccess) {
+                st = fail(GR_ERR_DEVICE, "graph launch");
+                break;
+            }
+            gr::DevControl ctl;
+            if ((st = read_ctrl(c, g->B.ctrl, g->cap, &ctl))) break;
+            if (ctl.unsupported) {
+                st = fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+                break;
+            }
+            if (!ctl.overflow) break;
+            if (attempt == 3) {
+                st = fail(GR_ERR_CAPACITY, "tile-list capacity could not be satisfied");
+                break;
+            }
+            g->refs_cap = std::max(g->refs_cap, grown(ctl.refs_needed));
+        }
+        if (st) {
+            graph_free(g);
+            return st;
+        }
+        c->graphs.push_back(g);
+        *out = g;
+        return GR_OK;
+    } catch (...) {
+        if (g) graph_free(g);
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_batch_graph_update(gr_batch_graph* g, const gr_object* objs, int32_t no) {
+    if (!g || (no > 0 && !objs)) return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+    if (no != (int32_t)g->objs.size())
+        return fail(GR_ERR_INVALID_ARGUMENT, "update must give the graph's number of objects");
+    for (int32_t k = 0; k < no; k++)
+        if (objs[k].mesh != g->objs[k].mesh)
+            return fail(GR_ERR_INVALID_ARGUMENT, "update must keep each object's mesh");
+    try {
+        gr_context* c = g->c;
+        GR_HIP(hipSetDevice(c->device));
+        g->objs.assign(objs, objs + no);
+        Plan P;
+        gr_status st = make_plan(c, g->bf.data(), (int)g->bf.size(), g->objs.data(), no, g->TW,
+                                 g->TH, g->frames.data(), g->dobj.data(), g->xcd.data(), P);
+        if (st) return st;
+        g->P = P;
+        // The header is read by the graph's last replay: wait for it first.
+        if (g->last) GR_HIP(hipEventSynchronize(g->done));
+        std::vector<unsigned char> h(g->P.header_bytes);
+        write_header(g->P, g->B, h.data());
+        GR_HIP(hipMemcpyAsync(g->B.header, h.data(), h.size(), hipMemcpyHostToDevice, g->cap));
+        GR_HIP(hipStreamSynchronize(g->cap));
+        return GR_OK;
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_batch_graph_launch(gr_batch_graph* g) {
+    if (!g) return fail(GR_ERR_INVALID_ARGUMENT, "null graph");
+    try {
+        GR_HIP(hipSetDevice(g->c->device));
+        return graph_launch(g);
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+void gr_batch_graph_destroy(gr_batch_graph* g) {
+    if (!g) return;
+    gr_context* c = g->c;
+    (void)hipSetDevice(c->device);
+    (void)sync_all(c);
+    (void)hipStreamSynchronize(g->cap);
+    c->graphs.erase(std::remove(c->graphs.begin(), c->graphs.end(), g), c->graphs.end());
+    graph_free(g);
+}
+
+gr_status gr_autotune(gr_context* c, const gr_batch_frame* frames, int32_t nf,
+                      const gr_object* objs, int32_t no, int32_t reps, gr_tune_result* results,
+                      int32_t max_results, int32_t* num_results, gr_config* best) {
+    if (!c || !frames || nf <= 0 || (no > 0 && !objs) || reps < 1)
+        return fail(GR_ERR_INVALID_ARGUMENT, "null argument, no frames or reps < 1");
+    if (num_results) *num_results = 0;
+    const gr_config saved = c->cfg;
+    const bool saved_one = c->one_pipe;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    unsigned long long* d_sum = nullptr;
+    struct Restore {
+        gr_context* c;
+        bool one;
+        hipEvent_t* a;
+        hipEvent_t* b;
+        unsigned long long** sum;
+        ~Restore() {
+            c->one_pipe = one;
+            if (*a) (void)hipEventDestroy(*a);
+            if (*b) (void)hipEventDestroy(*b);
+            if (*sum) (void)hipFree(*sum);
+        }
+    } restore{c, saved_one, &ea, &eb, &d_sum};
+    try {
+        GR_HIP(hipSetDevice(c->device));
+        gr_status st = sync_all(c);
+        if (st) return st;
+        GR_HIP(hipEventCreate(&ea));
+        GR_HIP(hipEventCreate(&eb));
+        GR_HIP(hipMalloc((void**)&d_sum, 8));
+        c->one_pipe = true;
+        static const int kTiles[4] = {8, 16, 32, 64};
+        static const int kSplits[4] = {256, 512, 1024, 4096};
+        bool have_best = false, have_sum = false;
+        double best_ms = 0.0;
+        unsigned long long first_sum = 0;
+        gr_config best_cfg = saved;
+        int32_t nres = 0;
+        for (int tw : kTiles)
+            for (int th : kTiles)
+                for (int split : kSplits) {
+                    c->cfg = saved;
+                    c->cfg.tile_w = tw;
+                    c->cfg.tile_h = th;
+                    c->cfg.tile_split = split;
+                    // Untimed launch: grows the tile lists while they overflow,
+                    // then sizes them to the batch's references plus a quarter.
+                    unsigned long long needed = 0;
+                    for (int attempt = 0;; attempt++) {
+                        if ((st = enqueue(c, frames, nf, objs, no))) return st;
+                        st = sync_and_check(c, &needed);
+                        if (st != GR_ERR_CAPACITY) break;
+                        if (attempt == 3)
+                            return fail(GR_ERR_CAPACITY, "tile-list capacity could not be satisfied");
+                        grow_refs(c, needed);
+                    }
+                    if (st) return st;
+                    std::vector<unsigned long long> sv;
+                    if ((st = read_stats(c, sv))) return st;
+                    unsigned long long refs = 0;
+                    for (int f = 0; f < nf; f++) refs += sv[(size_t)f * gr::kNumStats + gr::kStatRefs];
+                    c->cfg.refs_capacity = std::max<int64_t>(1 << 16, (int64_t)grown(refs));
+                    GR_HIP(hipEventRecord(ea, c->stream));
+                    for (int r = 0; r < reps; r++)
+                        if ((st = enqueue(c, frames, nf, objs, no))) return st;
+                    GR_HIP(hipEventRecord(eb, c->stream));
+                    if ((st = sync_and_check(c, &needed))) return st;
+                    float ms = 0.f;
+                    GR_HIP(hipEventElapsedTime(&ms, ea, eb));
+                    // Digest of every frame's colour and depth.
+                    gr::launch_fill32((uint32_t*)d_sum, 0u, 2, c->stream);
+                    for (int f = 0; f < nf; f++) {
+                        const size_t px = (size_t)frames[f].frame.width * frames[f].frame.height;
+                        gr::launch_checksum(frames[f].d_rgba, px, d_sum, c->stream);
+                        gr::launch_checksum(frames[f].d_depth, px, d_sum, c->stream);
+                    }
+                    GR_HIP(hipGetLastError());
+                    unsigned long long sum = 0;
+                    GR_HIP(hipMemcpyAsync(c->h_ctrl, d_sum, 8, hipMemcpyDeviceToHost, c->stream));
+                    GR_HIP(hipStreamSynchronize(c->stream));
+                    std::memcpy(&sum, c->h_ctrl, 8);
+                    const double per = (double)ms / reps;
+                    if (results && nres < max_results) {
+                        gr_tune_result& tr = results[nres];
+                        tr.tile_w = tw;
+                        tr.tile_h = th;
+                        tr.tile_split = split;
+                        tr.raster_waves = c->cfg.raster_waves;
+                        tr.ms_per_launch = per;
+                        tr.checksum = sum;
+                        nres++;
+                    }
+                    if (!have_sum) {
+                        first_sum = sum;
+                        have_sum = true;
+                    } else if (sum != first_sum) {
+                        c->cfg = saved;
+                        if (num_results) *num_results = nres;
+                        return fail(GR_ERR_DEVICE, "autotune: the image depends on the "
+                                                   "configuration (tile " +
+                                                       std::to_string(tw) + "x" +
+                                                       std::to_string(th) + ", split " +
+                                                       std::to_string(split) + ")");
+                    }
+                    if (!have_best || per < best_ms) {
+                        have_best = true;
+                        best_ms = per;
+                        best_cfg = c->cfg;
+                    }
+                }
+        best_cfg.pipelines = saved.pipelines;
+        c->cfg = best_cfg;
+        if (best) *best = best_cfg;
+        if (num_results) *num_results = nres;
+        return GR_OK;
+    } catch (...) {
+        c->cfg = saved;
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
 }
 
 }  // extern "C"

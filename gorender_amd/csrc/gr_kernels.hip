@@ -2142,6 +2142,28 @@ void launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
 }
 
+// Order-independent 64-bit digest of n words: the sum over i of a mix of
+// (word i, i), accumulated into *out (so one digest covers several buffers).
+__global__ __launch_bounds__(256) void k_checksum(const uint32_t* __restrict__ p, size_t n,
+                                                  unsigned long long* __restrict__ out) {
+    unsigned long long acc = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        unsigned long long x = ((unsigned long long)p[i] << 32) ^ (unsigned long long)i;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        acc += x ^ (x >> 31);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
+void launch_checksum(const void* p, size_t words, unsigned long long* out, hipStream_t s) {
+    if (words == 0) return;
+    const size_t blocks = std::min<size_t>((words + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_checksum, dim3((unsigned)blocks), dim3(256), 0, s, (const uint32_t*)p,
+                       words, out);
+}
+
 // Writes c.n words (at most kPutWords) carried by the kernel's arguments to
 // dst: host data reaches device memory without a host buffer that must outlive
 // the call, and without an allocation.

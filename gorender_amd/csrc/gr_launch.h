@@ -66,6 +66,8 @@ void launch_bin_fill(const LaunchParams& p, hipStream_t s);
 bool launch_raster_parts(const LaunchParams& p, hipStream_t s);
 bool launch_raster(const LaunchParams& p, hipStream_t s);
 void launch_fill32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// Adds an order-independent digest of `words` 32-bit words at p to *out.
+void launch_checksum(const void* p, size_t words, unsigned long long* out, hipStream_t s);
 // Copies `bytes` host bytes to dst (4-byte aligned) through kernel arguments.
 void launch_put(void* dst, const void* src, size_t bytes, hipStream_t s);
 

@@ -199,6 +199,27 @@ class Context:
         """One asynchronous batch: frame k draws scenes[k] into outputs[k]."""
         self.submit_batch(self.prepare_batch(scenes, mesh_handles, outputs))
 
+    def batch_graph(self, prepared) -> "BatchGraph":
+        """gr_batch_graph_create over a prepared batch (see prepare_batch)."""
+        return BatchGraph(self, prepared)
+
+    def autotune(self, prepared, reps: int = 3, max_results: int = 64):
+        """gr_autotune over a prepared batch: returns (best config as a dict,
+        every candidate tried as dicts); the context keeps the best config."""
+        frames, nf, objs, total = prepared
+        res = (_lib.gr_tune_result * max_results)()
+        n = ctypes.c_int32()
+        best = _lib.gr_config()
+        _lib.check(self.lib.gr_autotune(self.handle, frames, nf, objs, total, reps, res,
+                                        max_results, ctypes.byref(n), ctypes.byref(best)))
+        tried = [{"tile": [res[i].tile_w, res[i].tile_h], "tile_split": res[i].tile_split,
+                  "raster_waves": res[i].raster_waves,
+                  "ms_per_launch": round(res[i].ms_per_launch, 4),
+                  "checksum": f"{res[i].checksum:016x}"} for i in range(n.value)]
+        return ({"tile": [best.tile_w, best.tile_h], "tile_split": best.tile_split,
+                 "raster_waves": best.raster_waves, "refs_capacity": best.refs_capacity},
+                tried)
+
     def synchronize(self) -> None:
         _lib.check(self.lib.gr_synchronize(self.handle))
 
@@ -258,6 +279,12 @@ class DeviceBuffer:
         assert n <= self.nbytes and n <= src.nbytes
         _lib.check(self.ctx.lib.gr_device_copy(self.ctx.handle, self.ptr, src.ptr, n))
 
+    def checksum_into(self, digest: "DeviceBuffer", nbytes: int | None = None) -> None:
+        """Adds the digest of the first nbytes (all by default) to the 8-byte
+        device buffer `digest` (gr_device_checksum, on the context stream)."""
+        n = self.nbytes if nbytes is None else int(nbytes)
+        _lib.check(self.ctx.lib.gr_device_checksum(self.ctx.handle, self.ptr, n, digest.ptr))
+
     def to_numpy(self, dtype, shape) -> np.ndarray:
         out = np.empty(shape, dtype)
         assert out.nbytes <= self.nbytes
@@ -276,3 +303,32 @@ class DeviceBuffer:
         except Exception:
             pass
 
+
+class BatchGraph:
+    """One batch's launches captured in HIP graphs (gr_batch_graph_*)."""
+
+    def __init__(self, ctx: Context, prepared):
+        self.ctx = ctx
+        frames, nf, objs, total = prepared
+        self._prepared = prepared
+        h = ctypes.c_void_p()
+        _lib.check(ctx.lib.gr_batch_graph_create(ctx.handle, frames, nf, objs, total,
+                                                 ctypes.byref(h)))
+        self.handle = h.value
+
+    def launch(self) -> None:
+        _lib.check(self.ctx.lib.gr_batch_graph_launch(self.handle))
+
+    def update(self, objs, total: int) -> None:
+        _lib.check(self.ctx.lib.gr_batch_graph_update(self.handle, objs, total))
+
+    def destroy(self) -> None:
+        if self.handle and self.ctx.handle:
+            self.ctx.lib.gr_batch_graph_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

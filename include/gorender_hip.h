@@ -241,6 +241,51 @@ gr_status gr_draw_batch(gr_context *ctx, const gr_batch_frame *frames, int32_t n
 gr_status gr_draw_device_noalloc(gr_context *ctx, const gr_frame *frame,
                                  const gr_object *objects, int32_t num_objects, void *d_rgba,
                                  void *d_depth);
+/* ---- the batched form captured in HIP graphs (no reference counterpart) ----
+ * The launch sequence of one gr_draw_batch (counter zeroing, geometry and
+ * clipping, binning, raster) captured once, with device buffers of its own,
+ * in two HIP graphs (geometry + binning, then raster) that
+ * gr_batch_graph_launch replays: two graph launches per batch, no host work
+ * per frame. Meshes, textures and output buffers are bound at creation
+ * (destroying one the graph uses invalidates it); gr_batch_graph_update
+ * gives the same objects new matrices. Replays are ordered like gr_draw_batch
+ * calls: frames are written in submission order, and with two pipes a
+ * replay's geometry overlaps the previous batch's raster. A replay whose tile
+ * lists overflow makes gr_synchronize return GR_ERR_CAPACITY; the graph's next
+ * launch regrows them (draw it again). gr_last_stats then describes the
+ * graph's frames. */
+typedef struct gr_batch_graph gr_batch_graph;
+gr_status gr_batch_graph_create(gr_context *ctx, const gr_batch_frame *frames,
+                                int32_t num_frames, const gr_object *objects,
+                                int32_t num_objects, gr_batch_graph **out);
+/* New mvp/world matrices for the graph's objects (same count and meshes). */
+gr_status gr_batch_graph_update(gr_batch_graph *graph, const gr_object *objects,
+                                int32_t num_objects);
+gr_status gr_batch_graph_launch(gr_batch_graph *graph);
+void gr_batch_graph_destroy(gr_batch_graph *graph);
+
+/* ---- autotuner (no reference counterpart) ----
+ * Draws one batch under each candidate configuration and keeps the fastest in
+ * the context's config: raster tiles {8,16,32,64} x {8,16,32,64}, tile_split
+ * {256, 512, 1024, 4096}, with the tile-list capacity sized to what the batch
+ * needs plus a quarter. Each candidate is timed over `reps` launches (HIP
+ * events, one pipe) after one untimed launch, and its frames' colour and depth
+ * are checksummed on the device: the image may not depend on the
+ * configuration, so a candidate whose checksum differs from the first's fails
+ * the call (GR_ERR_DEVICE). results (may be NULL) receives up to max_results
+ * candidates in the order tried. */
+typedef struct {
+    int32_t tile_w, tile_h;
+    int32_t tile_split;
+    int32_t raster_waves;
+    double ms_per_launch;  /* mean over the timed launches */
+    uint64_t checksum;     /* of every frame's colour and depth buffers */
+} gr_tune_result;
+gr_status gr_autotune(gr_context *ctx, const gr_batch_frame *frames, int32_t num_frames,
+                      const gr_object *objects, int32_t num_objects, int32_t reps,
+                      gr_tune_result *results, int32_t max_results, int32_t *num_results,
+                      gr_config *best);
+
 /* Wait for queued draws; returns GR_ERR_CAPACITY if one of them overflowed its
  * bins (its output is then incomplete and must be drawn again). */
 gr_status gr_synchronize(gr_context *ctx);
@@ -262,6 +307,11 @@ gr_status gr_device_copy(gr_context *ctx, void *dst, const void *src, size_t byt
  * pixel, depth <= fog_end takes `color`, depths between blend toward it. */
 gr_status gr_fog(gr_context *ctx, void *rgba, const void *depth, int32_t width, int32_t height,
                  float fog_start, float fog_end, const uint8_t color[4]);
+/* Adds an order-independent 64-bit digest of `bytes` (a multiple of 4) device
+ * bytes at ptr to *digest_dev (device memory, 8 bytes), on the context stream
+ * after the draws queued before it: equal buffers give equal digests, so two
+ * forms of the same frames can be compared without copying them out. */
+gr_status gr_device_checksum(gr_context *ctx, const void *ptr, size_t bytes, void *digest_dev);
 /* Wait for the context stream, then copy device -> host. */
 gr_status gr_copy_to_host(gr_context *ctx, void *dst, const void *src, size_t bytes);
 

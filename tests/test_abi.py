@@ -43,7 +43,8 @@ def test_struct_layout_matches_c(tmp_path):
     """Compile the header as C99 and compare sizeof/offsetof with ctypes."""
     src = tmp_path / "sizes.c"
     structs = ["gr_texture_desc", "gr_mesh_desc", "gr_object", "gr_frame", "gr_stats",
-               "gr_batch_frame", "gr_config", "gr_server_config", "gr_serve_result"]
+               "gr_batch_frame", "gr_config", "gr_server_config", "gr_serve_result",
+               "gr_tune_result"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gorender_hip.h"',
              "int main(void) {"]
     for s in structs:
