@@ -393,16 +393,24 @@ def config2(nlat: int = 250, nlon: int = 400, width: int = 1280, height: int = 7
 
 def config3(total_faces: int = 1_000_000, width: int = 1920, height: int = 1080,
             instances: int = 64) -> SceneData:
-    """64 instances of icospheres L2..L6 and cubes in a 20^3 box, random unit
+    """64 instances of icospheres L2..L6 and cubes in a 20^3 box (levels raised
+    in seeded order until the faces reach at most `total_faces`), random unit
     quaternion rotation, scale [0.5, 3], textures 16^2..256^2, half of the meshes
     without vertex normals (flat), camera at the origin looking down -z."""
     rng = SplitMix64(SEED_BASE + 3)
     fr = FrameData(width, height)
     base = {lvl: icosphere(lvl) for lvl in range(2, 7)}
-    kinds = rng.randint(instances, 0, 5)
+    kinds = [int(v) for v in rng.randint(instances, 0, 5)]
+    # Weighted to the requested total: icospheres drawn in seeded order are
+    # raised one level (4x the faces) while the total stays within it.
+    faces = [12 if kd == 5 else 20 * 4 ** (kd + 2) for kd in kinds]
+    for i in [int(v) for v in rng.randint(4 * instances, 0, instances - 1)]:
+        if kinds[i] < 4 and sum(faces) + 3 * faces[i] <= total_faces:
+            kinds[i] += 1
+            faces[i] *= 4
     meshes, textures, objects = [], [], []
     for k in range(instances):
-        kind = int(kinds[k])
+        kind = kinds[k]
         src = cube() if kind == 5 else base[kind + 2]
         m = MeshData(src.vertices.copy(), src.vertex_indices.copy(),
                      src.vertex_normals.copy(), src.normal_indices.copy(), src.uvs.copy())

@@ -5,6 +5,7 @@ Run from the repository root after building the oracle (make -C oracle):
     python tests/golden/make_golden.py              # every scene
     python tests/golden/make_golden.py NAME ...     # only these (others kept)
     python tests/golden/make_golden.py --missing    # only scenes without a digest
+    python tests/golden/make_golden.py --jobs 8 ... # oracle runs in 8 processes
 The digests pin the oracle and the seeded scene generators against drift; the
 GPU parity tests compare the HIP path with them as well as with the live oracle.
 """
@@ -22,9 +23,22 @@ from parity import digest  # noqa: E402
 from scene_cases import golden_cases  # noqa: E402
 
 
+_BUILDERS = {}
+
+
+def _one(name):
+    scene = _BUILDERS[name]()
+    rgba, depth, st = refcpu.draw(scene)
+    return name, digest(rgba, depth, st)
+
+
 def main():
     path = os.path.join(HERE, "oracle_digests.json")
     args = sys.argv[1:]
+    jobs = 1
+    if args[:1] == ["--jobs"]:
+        jobs = int(args[1])
+        args = args[2:]
     old = json.load(open(path)) if args and os.path.exists(path) else {}
     cases = golden_cases()
     names = {n for n, _ in cases}
@@ -38,13 +52,19 @@ def main():
     else:
         todo = names
     out = {n: d for n, d in old.items() if n in names}
-    for name, build in cases:
-        if name not in todo:
-            continue
-        scene = build()
-        rgba, depth, st = refcpu.draw(scene)
-        out[name] = digest(rgba, depth, st)
-        print(name, out[name]["triangles"], out[name]["pixels_shaded"], flush=True)
+    _BUILDERS.update(cases)
+    order = [n for n, _ in cases if n in todo]
+    if jobs > 1:
+        import multiprocessing
+        with multiprocessing.get_context("fork").Pool(jobs) as pool:
+            results = pool.imap_unordered(_one, order)
+            for name, d in results:
+                out[name] = d
+                print(name, d["triangles"], d["pixels_shaded"], flush=True)
+    else:
+        for name in order:
+            name, out[name] = _one(name)
+            print(name, out[name]["triangles"], out[name]["pixels_shaded"], flush=True)
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
         f.write("\n")

@@ -278,24 +278,38 @@ def all_cases():
     ] + [(f"hand-{k}", (lambda k=k: hand_frame(k))) for k in HAND_KINDS]
 
 
-@functools.lru_cache(maxsize=1)
+CONFIG4_FRAMES = 256   # the batch of config 4
+CONFIG5_REQUESTS = 64  # the distinct requests the serving benchmark replays
+
+
+@functools.lru_cache(maxsize=2)
+def config4_all():
+    """All frames of config 4 (the first n of them are config4_frames(n))."""
+    return S.config4_frames(CONFIG4_FRAMES)
+
+
 def config4_batch(n=16):
-    return S.config4_frames(n)
+    return config4_all()[:n]
 
 
 @functools.lru_cache(maxsize=1)
+def config5_all():
+    return S.config5(CONFIG5_REQUESTS)
+
+
 def config5_set(n=8):
-    return S.config5(n)
+    rs = config5_all()
+    return S.ResidentSet(rs.meshes, rs.textures, rs.requests[:n])
 
 
 def golden_cases():
     """Every scene with a committed oracle digest under tests/golden: the small
-    cases, configs 1-3, the first 16 frames of config 4 and the first 8 requests
-    of config 5 (as the subscene of the meshes each one draws)."""
+    cases, configs 1-3, all 256 frames of config 4 and all 64 requests of
+    config 5 (as the subscene of the meshes each one draws)."""
     cases = list(all_cases())
     cases += [("config1-ref", lambda: S.config1("ref")), ("config1-cfg", lambda: S.config1("cfg")),
               ("config2", S.config2), ("config3", S.config3)]
-    cases += [(f"config4-{k}", (lambda k=k: config4_batch()[k])) for k in range(16)]
-    cases += [(f"config5-{k}", (lambda k=k: S.subscene(config5_set().requests[k])))
-              for k in range(8)]
+    cases += [(f"config4-{k}", (lambda k=k: config4_all()[k])) for k in range(CONFIG4_FRAMES)]
+    cases += [(f"config5-{k}", (lambda k=k: S.subscene(config5_all().requests[k])))
+              for k in range(CONFIG5_REQUESTS)]
     return cases

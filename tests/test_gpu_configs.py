@@ -17,7 +17,7 @@ from gorender_amd import _lib
 from gorender_amd import scenes as S
 from oracle import refcpu
 from parity import assert_same, digest
-from scene_cases import config4_batch, config5_set
+from scene_cases import config4_all, config4_batch, config5_all, config5_set
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +79,31 @@ def test_config5_resident(ctx):
         check(f"config5-{k}", req, ctx.draw(req, mh), S.subscene(req))
     for k, got in enumerate(draw_batch(ctx, rs.requests, [mh] * len(rs.requests))):
         check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
+
+
+def test_config4_256_frames_one_batch(ctx):
+    """Config 4 at its stated size: all 256 frames (1k-500k triangles, four
+    resolutions, half textured) in ONE gr_draw_batch, each frame against the
+    oracle's committed digest."""
+    frames = config4_all()
+    assert len(frames) == 256
+    handles = [ctx.upload_scene(s)[0] for s in frames]
+    print(f"config4: {sum(s.num_faces for s in frames)} triangles in one batch", flush=True)
+    for k, got in enumerate(draw_batch(ctx, frames, handles)):
+        check(f"config4-{k}", frames[k], got)
+
+
+def test_config5_all_requests(ctx):
+    """All 64 requests of config 5 over the 32 resident meshes, as four batches
+    of 16, against the oracle's digests of each request drawn alone."""
+    rs = config5_all()
+    assert len(rs.requests) == 64
+    th = [ctx.upload_texture(t) for t in rs.textures]
+    mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
+    for b in range(4):
+        reqs = rs.requests[16 * b:16 * (b + 1)]
+        for k, got in enumerate(draw_batch(ctx, reqs, [mh] * len(reqs))):
+            check(f"config5-{16 * b + k}", reqs[k], got, S.subscene(reqs[k]))
 
 
 @pytest.mark.parametrize("tile,split", [((32, 32), 64), ((16, 16), 64), ((64, 64), 64)])

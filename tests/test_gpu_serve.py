@@ -13,7 +13,8 @@ from gorender_amd import _lib
 from gorender_amd import scenes as S
 from gorender_amd.serve import PreparedRequest, Server, poisson_arrivals, replay
 from parity import digest
-from scene_cases import config5_set
+from gorender_amd import scenes as S
+from scene_cases import config5_all
 
 pytestmark = pytest.mark.gpu
 
@@ -21,11 +22,19 @@ GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "oracl
 
 
 @pytest.fixture(scope="module")
-def resident(ctx):
-    rs = config5_set()
+def resident_all(ctx):
+    """The 32 resident meshes and all 64 requests the serving benchmark replays."""
+    rs = config5_all()
     th = [ctx.upload_texture(t) for t in rs.textures]
     mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
     return rs, [PreparedRequest(r, mh) for r in rs.requests]
+
+
+@pytest.fixture(scope="module")
+def resident(resident_all):
+    """The first 8 requests over the same resident meshes."""
+    rs, reqs = resident_all
+    return S.ResidentSet(rs.meshes, rs.textures, rs.requests[:8]), reqs[:8]
 
 
 def check_result(k, res):
@@ -54,6 +63,20 @@ def test_mix_equals_alone(ctx, resident, budget_mib, max_batch, max_inflight):
     if max_batch > 1:
         assert max(sizes) > 1, "requests in flight together were never batched"
     for k, res in zip(order, results):
+        check_result(k, res)
+
+
+def test_all_benchmark_requests(ctx, resident_all):
+    """The 64 distinct requests of the serving benchmark, served at its
+    settings (2 GiB budget, batches of up to 16, 2 launches in flight), each
+    equal to the oracle's digest of that request drawn alone."""
+    rs, reqs = resident_all
+    assert len(reqs) == 64
+    with Server(ctx, 2 << 30, 16, 2) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    assert max(r.batch_size for r in results) > 1
+    for k, res in enumerate(results):
         check_result(k, res)
 
 
