@@ -130,6 +130,7 @@ SIGNATURES = {
     "gr_matrix_screen": (None, [c_int32, c_int32, c_void_p]),
     "gr_matrix_mvp": (None, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "gr_vec3_normalize": (None, [c_void_p, c_void_p]),
+    "gr_box_visibility": (c_int32, [c_void_p, c_void_p, c_float, c_float]),
     "gr_draw": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32, c_void_p, c_void_p,
                           POINTER(gr_stats)]),
     "gr_draw_device": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32, c_void_p,
@@ -144,6 +145,9 @@ SIGNATURES = {
     "gr_batch_graph_destroy": (None, [c_void_p]),
     "gr_autotune": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_int32,
                               c_void_p, c_int32, POINTER(c_int32), POINTER(gr_config)]),
+    "gr_present_draw": (c_int32, [c_void_p, POINTER(gr_frame), c_void_p, c_int32, c_int32]),
+    "gr_present_wait": (c_int32, [c_void_p, POINTER(ctypes.POINTER(c_uint8)),
+                                  POINTER(ctypes.POINTER(ctypes.c_float)), POINTER(gr_stats)]),
     "gr_synchronize": (c_int32, [c_void_p]),
     "gr_last_stats": (c_int32, [c_void_p, c_void_p, c_int32, POINTER(c_int32)]),
     "gr_device_alloc": (c_int32, [c_void_p, ctypes.c_size_t, POINTER(c_void_p)]),

@@ -113,6 +113,23 @@ struct gr_context {
     gr::DevControl* noalloc_ctrl = nullptr;
     // Live batch graphs (their control words are checked by gr_synchronize).
     std::vector<gr_batch_graph*> graphs;
+    const uint32_t* last_overflow = nullptr;  // device overflow flags of the last draw's frames
+
+    // The present path (gr_present_draw / gr_present_wait): two frames in
+    // flight, each drawn into device buffers and copied to pinned host memory.
+    struct PresentSlot {
+        DevBuf rgba, depth;
+        void* host = nullptr;     // pinned: colour, then depth
+        size_t host_cap = 0;
+        void* h_meta = nullptr;   // pinned: stats of the frame, then its overflow flag
+        hipEvent_t copied = nullptr;
+        bool pending = false;
+        bool with_depth = false;
+        gr_frame frame{};
+        std::vector<gr_object> objs;
+    } present[2];
+    int present_next = 0;  // slot of the next gr_present_draw
+    int present_head = 0;  // oldest queued slot
 
     gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 0};
 
@@ -462,6 +479,7 @@ gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_ob
             o.frame = f;
             o.gouraud = (fr.flags & GR_LIGHTING) && !(fr.flags & GR_FLAT_SHADING) && mr.dm.N > 0;
             o.dense = dense_on_screen(ob.mvp, mr.dm.bbox, mr.dm.F, fr.width, fr.height) ? 1 : 0;
+            o.vis = gr_box_visibility(ob.mvp, &mr.dm.bbox[0][0], fr.znear, fr.zfar);
             o.face_blocks = (mr.dm.F + gr::kFacesPerWg - 1) / gr::kFacesPerWg;
             o.face_block = P.fblocks;
             o.seq_block = seq;
@@ -723,6 +741,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     c->last_pipe = pi;
     c->last_frames = nf;
     c->last_stats = p.stats;
+    c->last_overflow = p.frame_overflow;
     return GR_OK;
 }
 
@@ -818,6 +837,32 @@ void free_mesh(MeshRes& mr) {
 }
 
 bool is_pow2(int32_t x) { return x != 0 && (x & (x - 1)) == 0; }
+
+// Queues the draw of a present slot's frame and the copies of its colour (and
+// depth), stats and overflow flag to pinned host memory, on the stream of the
+// pipe that draws it (the next frame runs on the other pipe meanwhile).
+gr_status present_enqueue(gr_context* c, gr_context::PresentSlot& ps) {
+    const size_t npx = (size_t)ps.frame.width * ps.frame.height;
+    gr_batch_frame bf;
+    bf.frame = ps.frame;
+    bf.first_object = 0;
+    bf.num_objects = (int32_t)ps.objs.size();
+    bf.d_rgba = ps.rgba.p;
+    bf.d_depth = ps.depth.p;
+    gr_status st = enqueue(c, &bf, 1, ps.objs.data(), (int)ps.objs.size());
+    if (st) return st;
+    const hipStream_t s = c->pipes[c->last_pipe].stream;
+    GR_HIP(hipMemcpyAsync(ps.host, ps.rgba.p, npx * 4, hipMemcpyDeviceToHost, s));
+    if (ps.with_depth)
+        GR_HIP(hipMemcpyAsync((char*)ps.host + npx * 4, ps.depth.p, npx * 4,
+                              hipMemcpyDeviceToHost, s));
+    GR_HIP(hipMemcpyAsync(ps.h_meta, c->last_stats, gr::kNumStats * 8, hipMemcpyDeviceToHost, s));
+    GR_HIP(hipMemcpyAsync((char*)ps.h_meta + gr::kNumStats * 8, c->last_overflow, 4,
+                          hipMemcpyDeviceToHost, s));
+    GR_HIP(hipEventRecord(ps.copied, s));
+    ps.pending = true;
+    return GR_OK;
+}
 
 void free_pipe_bufs(Pipe& pp) {
     DevBuf* pb[] = {&pp.batch, &pp.rrec,    &pp.srec, &pp.wrec,  &pp.tile_counts,
@@ -1219,6 +1264,13 @@ void gr_destroy(gr_context* c) {
         free_pipe_bufs(pp);
         if (pp.raster_done) (void)hipEventDestroy(pp.raster_done);
     }
+    for (auto& ps : c->present) {
+        if (ps.rgba.p) (void)hipFree(ps.rgba.p);
+        if (ps.depth.p) (void)hipFree(ps.depth.p);
+        if (ps.host) (void)hipHostFree(ps.host);
+        if (ps.h_meta) (void)hipHostFree(ps.h_meta);
+        if (ps.copied) (void)hipEventDestroy(ps.copied);
+    }
     if (c->own1) (void)hipStreamDestroy(c->own1);
     for (auto& s : c->ring) {
         if (s.host) (void)hipHostFree(s.host);
@@ -1271,10 +1323,6 @@ gr_status gr_set_stream(gr_context* c, void* s) {
 }
 
 void* gr_get_stream(gr_context* c) { return c ? (void*)c->stream : nullptr; }
-
-int32_t gr_texture_kind_for_size(int32_t w, int32_t h) {
-    return (is_pow2(w) && is_pow2(h)) ? GR_TEXTURE_IMAGE_FAST : GR_TEXTURE_IMAGE;
-}
 
 gr_status gr_texture_create(gr_context* c, const gr_texture_desc* d, const uint8_t* rgba,
                             int32_t* out) {
@@ -1624,6 +1672,83 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
             return GR_OK;
         }
         return fail(GR_ERR_CAPACITY, "tile-list capacity could not be satisfied");
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_present_draw(gr_context* c, const gr_frame* frame, const gr_object* objs,
+                          int32_t no, int32_t with_depth) {
+    try {
+        if (!c || !frame || no < 0 || (no > 0 && !objs))
+            return fail(GR_ERR_INVALID_ARGUMENT, "null argument");
+        if (with_depth != 0 && with_depth != 1)
+            return fail(GR_ERR_INVALID_ARGUMENT, "with_depth must be 0 or 1");
+        GR_HIP(hipSetDevice(c->device));
+        gr_context::PresentSlot& ps = c->present[c->present_next];
+        if (ps.pending)
+            return fail(GR_ERR_INVALID_ARGUMENT,
+                        "two frames in flight: gr_present_wait for the oldest first");
+        gr_status st = validate_frame(*frame);
+        if (st) return st;
+        const size_t npx = (size_t)frame->width * frame->height;
+        if ((st = ensure(c, ps.rgba, npx * 4)) || (st = ensure(c, ps.depth, npx * 4))) return st;
+        const size_t hb = npx * (with_depth ? 8 : 4);
+        if (ps.host_cap < hb) {
+            if (ps.host) GR_HIP(hipHostFree(ps.host));
+            ps.host = nullptr;
+            ps.host_cap = 0;
+            GR_HIP(hipHostMalloc(&ps.host, hb, hipHostMallocDefault));
+            ps.host_cap = hb;
+        }
+        if (!ps.h_meta) GR_HIP(hipHostMalloc(&ps.h_meta, gr::kNumStats * 8 + 16, hipHostMallocDefault));
+        if (!ps.copied) GR_HIP(hipEventCreateWithFlags(&ps.copied, hipEventDisableTiming));
+        ps.frame = *frame;
+        ps.objs.assign(objs, objs + no);
+        ps.with_depth = with_depth != 0;
+        if ((st = present_enqueue(c, ps))) return st;
+        c->present_next ^= 1;
+        return GR_OK;
+    } catch (...) {
+        return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+gr_status gr_present_wait(gr_context* c, const uint8_t** rgba, const float** depth,
+                          gr_stats* stats) {
+    try {
+        if (!c) return fail(GR_ERR_INVALID_ARGUMENT, "null context");
+        GR_HIP(hipSetDevice(c->device));
+        gr_context::PresentSlot& ps = c->present[c->present_head];
+        if (!ps.pending) return fail(GR_ERR_INVALID_ARGUMENT, "no frame queued");
+        for (int attempt = 0;; attempt++) {
+            GR_HIP(hipEventSynchronize(ps.copied));
+            uint32_t over = 0;
+            std::memcpy(&over, (char*)ps.h_meta + gr::kNumStats * 8, 4);
+            if (!over) break;
+            // The frame's tile lists overflowed: grow them and draw it again
+            // (after every queued draw, so the next frame keeps its order).
+            unsigned long long needed = 0;
+            gr_status st = sync_and_check(c, &needed);
+            if (st && st != GR_ERR_CAPACITY) return st;
+            if (attempt == 3)
+                return fail(GR_ERR_CAPACITY, "tile-list capacity could not be satisfied");
+            grow_refs(c, needed);
+            if ((st = present_enqueue(c, ps))) return st;
+        }
+        ps.pending = false;
+        c->present_head ^= 1;
+        const unsigned long long* q = (const unsigned long long*)ps.h_meta;
+        if (stats) to_gr_stats(q, stats);
+        if (rgba) *rgba = (const uint8_t*)ps.host;
+        if (depth)
+            *depth = ps.with_depth
+                         ? (const float*)((const char*)ps.host +
+                                          (size_t)ps.frame.width * ps.frame.height * 4)
+                         : nullptr;
+        if (q[gr::kStatLongLines])
+            return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
+        return GR_OK;
     } catch (...) {
         return fail(GR_ERR_OUT_OF_MEMORY, "host allocation failed");
     }

@@ -65,6 +65,7 @@ struct DevObject {
     int32_t face_blocks;    // number of k_face blocks
     int32_t seq_block;      // first face block index within the frame (draw order)
     int32_t dense;          // >= 1 face per 4 pixels of the projected bounding box
+    int32_t vis;            // BoxVisibility of the object (kBoxOutside/Intersect/Inside)
 };
 
 struct DevTexture {

@@ -92,6 +92,40 @@ M4 load(const float* in) {
     return r;
 }
 
+// BoxVisibility of an object (renderer.go:268-275, clipping.go:131-154): its
+// bounding-box corners through MVP in the transform's order
+// (asm_purego.go:9-19), then per plane in Left, Right, Top, Bottom, Near, Far
+// order the corners with N.c - N.P > 0 (clipping.go:69-71) are outside; the
+// first plane with every corner outside gives Outside, the first with some
+// outside gives Intersect, none gives Inside (0, 1, 2). float32 throughout,
+// no contraction.
+int box_visibility(const float* m, const float bbox[8][4], float zn, float zf) {
+    float c[8][4];
+    for (int k = 0; k < 8; k++) {
+        const float x = bbox[k][0], y = bbox[k][1], z = bbox[k][2], w = bbox[k][3];
+        for (int r = 0; r < 4; r++)
+            c[k][r] = m[4 * r] * x + m[4 * r + 1] * y + m[4 * r + 2] * z + m[4 * r + 3] * w;
+    }
+    // clipping.go:93-129: (point, normal) per plane.
+    const float P[6][4] = {{-1, 0, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 1},
+                           {0, 1, 0, 1},  {0, 0, zn, 1}, {0, 0, zf, 1}};
+    const float N[6][4] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1},
+                           {0, -1, 0, 1}, {0, 0, -1, 0}, {0, 0, 1, 0}};
+    for (int p = 0; p < 6; p++) {
+        const float np = N[p][0] * P[p][0] + N[p][1] * P[p][1] + N[p][2] * P[p][2] +
+                         N[p][3] * P[p][3];
+        int outside = 0;
+        for (int k = 0; k < 8; k++) {
+            const float d = N[p][0] * c[k][0] + N[p][1] * c[k][1] + N[p][2] * c[k][2] +
+                            N[p][3] * c[k][3];
+            outside += d - np > 0.f;
+        }
+        if (outside == 8) return 0;
+        if (outside > 0) return 1;
+    }
+    return 2;
+}
+
 }  // namespace
 
 extern "C" {
@@ -189,3 +223,15 @@ void gr_vec3_normalize(const float in[3], float out[3]) {
 }
 
 }  // extern "C"
+
+extern "C" int32_t gr_box_visibility(const float* mvp, const float* bbox, float znear, float zfar) {
+    if (!mvp || !bbox) return -1;
+    return box_visibility(mvp, reinterpret_cast<const float(*)[4]>(bbox), znear, zfar);
+}
+
+// NewImageTexture picks the fast (power-of-two) type when both sizes are
+// powers of two (texture.go:40-42, isPowerOfTwo utils.go:7-9).
+extern "C" int32_t gr_texture_kind_for_size(int32_t w, int32_t h) {
+    auto pow2 = [](int32_t x) { return x != 0 && (x & (x - 1)) == 0; };
+    return (pow2(w) && pow2(h)) ? GR_TEXTURE_IMAGE_FAST : GR_TEXTURE_IMAGE;
+}

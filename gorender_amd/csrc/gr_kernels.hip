@@ -18,6 +18,7 @@
 // uses -ffp-contract=off and correctly rounded division/sqrt.
 #pragma clang fp contract(off)
 
+
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
@@ -227,6 +228,21 @@ __device__ __forceinline__ int last_le(int n, int b, F key) {
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// The wave's total of a per-lane count that only lane 0 holds in the common
+// case (the raster's covered-fragment count is wave-uniform except on the
+// int64 path): the full reduction runs only when another lane holds a count.
+__device__ __forceinline__ unsigned long long wave_total(unsigned long long v, int lane) {
+    if (__ballot(v != 0ull && lane != 0) == 0ull)
+        return __builtin_amdgcn_readfirstlane((unsigned)v) |
+               ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32)) << 32);
+    return wave_sum_u64(v);
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
@@ -847,25 +863,6 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total
     return base + x - v;
 }
 
-// renderer.go:268-275 + clipping.go:131-154: lanes 0..7 transform one box
-// corner each; per plane the outside corners are counted by ballot. Uniform.
-__device__ int box_visibility(const DevObject& o, const DevFrame& fr, int lane) {
-    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < 8)
-        c = xform(o.mvp, o.m.bbox[lane][0], o.m.bbox[lane][1], o.m.bbox[lane][2],
-                  o.m.bbox[lane][3]);
-    for (int p = 0; p < 6; p++) {
-        float4 P, N;
-        plane_of(p, fr.znear, fr.zfar, P, N);
-        float np = N.x * P.x + N.y * P.y + N.z * P.z + N.w * P.w;
-        float d = N.x * c.x + N.y * c.y + N.z * c.z + N.w * c.w;
-        int outside = __popcll(__ballot(lane < 8 && d - np > 0.f));
-        if (outside == 8) return kBoxOutside;
-        if (outside > 0) return kBoxIntersect;
-    }
-    return kBoxInside;
-}
-
 // Clip-space corner k of a face: MVP * (x, y, z, w) (renderer.go:303-304,318-320).
 __device__ __forceinline__ float4 corner(const DevObject& o, int v) {
     const float* p = o.m.pos + 3 * (size_t)v;
@@ -888,7 +885,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                               int thl, const int* __restrict__ xcd_face,
                                               int nframes) {
     __shared__ int lds_scan[4];
-    __shared__ unsigned long long lds_st[4][4];
+    // Per-wave counters: TPF references | binned triangles << 16, front faces
+    // | clip overflows << 16 (each field stays below 2^16 for 64 faces).
+    __shared__ uint32_t lds_cnt[4][2];
     __shared__ int lds_nsurv[4];                     // survivors per wave
     __shared__ uint32_t lds_sface[kFacesPerWg];      // survivor: face offset | plane << 8
     __shared__ float lds_spos[kFacesPerWg][12];      // survivor: clip-space corners
@@ -918,7 +917,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const int face = wg_in_obj * kFacesPerWg + threadIdx.x;
     const uint32_t flags = fr.flags;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int vis = box_visibility(o, fr, lane);
+    const int vis = o.vis;  // BoxVisibility, per object on the host
 
     FaceStats st = {0u};
 
@@ -1110,23 +1109,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     if (threadIdx.x == 0) wg_count[b] = (uint32_t)total;
 
-    // Block-reduce the counters, one atomic per counter per block.
+    // Counters per wave: front faces and clip overflows are one bit per lane
+    // (ballot counts), TPF references and binned triangles one 32-bit sum;
+    // then one global atomic per counter per workgroup.
     {
-        unsigned long long s0 = wave_sum_u64(st.packed & 0xFFFFu);
-        unsigned long long s1 = wave_sum_u64((st.packed >> 16) & 0xFFu);
-        unsigned long long s2 = wave_sum_u64((st.packed >> 24) & 0xFu);
-        unsigned long long s3 = wave_sum_u64(st.packed >> 28);
+        const uint32_t c0 = wave_sum_u32((st.packed & 0xFFFFu) | (((st.packed >> 16) & 0xFFu) << 16));
+        const uint32_t nf = (uint32_t)__popcll(__ballot(((st.packed >> 24) & 0xFu) != 0u));
+        const uint32_t nc = (uint32_t)__popcll(__ballot((st.packed >> 28) != 0u));
         if (lane == 0) {
-            lds_st[wave][0] = s0;
-            lds_st[wave][1] = s1;
-            lds_st[wave][2] = s2;
-            lds_st[wave][3] = s3;
+            lds_cnt[wave][0] = c0;
+            lds_cnt[wave][1] = nf | (nc << 16);
         }
     }
     __syncthreads();
     if (threadIdx.x < 4) {
-        unsigned long long t = lds_st[0][threadIdx.x] + lds_st[1][threadIdx.x] +
-                               lds_st[2][threadIdx.x] + lds_st[3][threadIdx.x];
+        const int wd = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 16;
+        const unsigned long long t = ((lds_cnt[0][wd] >> sh) & 0xFFFFu) + ((lds_cnt[1][wd] >> sh) & 0xFFFFu) +
+                                     ((lds_cnt[2][wd] >> sh) & 0xFFFFu) + ((lds_cnt[3][wd] >> sh) & 0xFFFFu);
         const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
         if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
     }
@@ -1661,36 +1660,61 @@ __device__ __forceinline__ unsigned long long raster_list(
             // (y * TW + x, below 4096 for every tile size), exact in every lane
             // that enqueues (its pixel lies inside the tile).
             uint32_t prow = (i << 12) + (uint32_t)(py - Y0) * (uint32_t)TW + (uint32_t)(px - X0);
+            // One walk step: lanes with all < 0 hold an inside fragment (every
+            // edge value < 0, rasterizer.go:147, and the pixel inside the box);
+            // they go to the wave's ring by ballot, and every 64 queued
+            // fragments get their exact depth.
+            auto step = [&](int all, int g1, int g2, uint32_t pk) {
+                const uint64_t mask = __ballot(all < 0);
+                if (mask) {
+                    if (all < 0) {
+                        const uint32_t k =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo(
+                                                          (uint32_t)mask, qtail)) &
+                            127u;
+                        q[0][k] = pk;
+                        q[1][k] = (uint32_t)g1;
+                        q[2][k] = (uint32_t)g2;
+                    }
+                    const uint32_t m = (uint32_t)__popcll(mask);
+                    qtail += m;
+                    covered_wave += m;
+                    if (qtail - qhead >= 64u) {
+                        frag_depth<TW>(q, qhead, 64u, lrec, lslot, key, lane);
+                        qhead += 64u;
+                    }
+                }
+            };
+            if (x1 - x0 < lpr) {
+                // The box is one column block wide (the common case): a flat
+                // loop over row blocks. The lane's column is inside the box
+                // iff gx < 0; only the last block can hold rows beyond it.
+                const int gx = lx - (x1 - x0) - 1;
+                const int nfull = (y1 - y0 + 1) >> (6 - l2);  // blocks of `rows` whole rows
+                uint32_t pk = prow;
+                for (int sb = 0; sb < nfull; sb++) {
+                    step(f0 & f1 & f2 & gx, f1, f2, pk);
+                    f0 += sy0;
+                    f1 += sy1;
+                    f2 += sy2;
+                    pk += (uint32_t)rows * (uint32_t)TW;
+                }
+                const int rest = (y1 - y0 + 1) - (nfull << (6 - l2));
+                if (rest > 0) {
+                    const int gy = ly - rest;  // negative iff the lane's row is inside
+                    step(f0 & f1 & f2 & gx & gy, f1, f2, pk);
+                }
+                continue;
+            }
             for (int yb = y0; yb <= y1; yb += rows) {
                 // Negative exactly when the lane's row is inside the box.
                 const int gy = ly - (y1 - yb) - 1;
                 int g0 = f0, g1 = f1, g2 = f2;
                 uint32_t pk = prow;
                 for (int xb = x0; xb <= x1; xb += lpr) {
-                    // All five negative: the pixel is inside the box and every
-                    // edge value is < 0 (rasterizer.go:147).
                     const int gx = lx - (x1 - xb) - 1;
-                    const int all = g0 & g1 & g2 & gx & gy;
-                    const uint64_t mask = __ballot(all < 0);
-                    if (mask) {
-                        if (all < 0) {
-                            const uint32_t k =
-                                __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo(
-                                                              (uint32_t)mask, qtail)) &
-                                127u;
-                            q[0][k] = pk;
-                            q[1][k] = (uint32_t)g1;
-                            q[2][k] = (uint32_t)g2;
-                        }
-                        const uint32_t m = (uint32_t)__popcll(mask);
-                        qtail += m;
-                        covered_wave += m;
-                        if (qtail - qhead >= 64u) {
-                            frag_depth<TW>(q, qhead, 64u, lrec, lslot, key, lane);
-                            qhead += 64u;
-                        }
-                    }
+                    step(g0 & g1 & g2 & gx & gy, g1, g2, pk);
                     g0 += sx0;
                     g1 += sx1;
                     g2 += sx2;
@@ -1712,7 +1736,7 @@ __device__ __forceinline__ unsigned long long raster_list(
 }
 
 // Deferred shading of the winning fragment of each pixel of the tile, one
-// write per pixel. Returns this thread's count of shaded pixels.
+// write per pixel. Returns the wave's count of shaded pixels.
 //
 // Background colours are read as values and the dot test is evaluated
 // without short-circuit branches: hipcc 7.2 (clang 22) turned the
@@ -1726,12 +1750,14 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
     const uint32_t dot_color = fr.dot_color, clear_color = fr.clear_color;
     const int step = fr.dot_step;
     const int step1 = step >= 1 ? step : 1;
-    unsigned long long shaded = 0;
+    unsigned long long shaded = 0;  // wave-uniform
     for (int i = threadIdx.x; i < NPX; i += 256) {
         const int x = X0 + (i % TW), y = Y0 + (i / TW);
-        if (x >= fr.W || y >= fr.H) continue;
+        const bool inframe = x < fr.W && y < fr.H;
         const unsigned long long k = key[i];
         const uint32_t seq = (uint32_t)k;
+        shaded += (unsigned long long)__popcll(__ballot(inframe && seq != 0u));
+        if (!inframe) continue;
         uint32_t color;
         float depth;
         if (seq == 0u) {
@@ -1742,7 +1768,6 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             color = dot ? dot_color : clear_color;
             depth = -1.0f;
         } else {
-            shaded++;
             const uint32_t slot = seq - 1u;
             const RasterRec r = gload(&fr.rrec[slot]);
             const float4* rp = reinterpret_cast<const float4*>(&r);
@@ -1822,7 +1847,7 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
     __shared__ uint4 lrec[kRasterChunk * 5];
     __shared__ uint32_t lslot[kRasterChunk];
     __shared__ uint32_t qbuf[4][3][128];
-    __shared__ unsigned long long red[4];
+    __shared__ unsigned long long red;  // covered fragments of the item
     __shared__ RasterItem litem;
     const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
     // Written by k_bin_scan (an earlier kernel).
@@ -1845,7 +1870,10 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
             }
             it = (uint32_t)sgpr((int)it);
             const RasterItem w = it < n_items ? items[it] : RasterItem{kNoSlab, 0u};
-            if (lane == 0) litem = w;
+            if (lane == 0) {
+                litem = w;
+                red = 0ull;
+            }
         }
         __syncthreads();
         const uint32_t wtile = (uint32_t)sgpr((int)litem.tile);
@@ -1870,13 +1898,10 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
             const unsigned long long k = key[i];
             if ((uint32_t)k != 0u) atomicMax(&slab[i], k);
         }
-        covered = wave_sum_u64(covered);
-        if (lane == 0) red[wave] = covered;
+        const unsigned long long cov = wave_total(covered, lane);
+        if (lane == 0 && cov) atomicAdd(&red, cov);
         __syncthreads();
-        if (wave == 0) {
-            const unsigned long long c = red[0] + red[1] + red[2] + red[3];
-            if (lane == 0 && c) atomicAdd(&stats[fi * kNumStats + kStatCovered], c);
-        }
+        if (wave == 0 && lane == 0 && red) atomicAdd(&stats[fi * kNumStats + kStatCovered], red);
         // litem and red are read before the next item overwrites them.
         __syncthreads();
     }
@@ -1896,7 +1921,7 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
     __shared__ uint4 lrec[kRasterChunk * 5];  // the staged raster records
     __shared__ uint32_t lslot[kRasterChunk];
     __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
-    __shared__ unsigned long long red[4][2];
+    __shared__ unsigned long long red[4][2];  // per wave: covered fragments, shaded pixels
     const int g = blockIdx.x;
     const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
     const DevFrame& fr = frames[fi];
@@ -1928,8 +1953,8 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
     // accumulates into them again.
     if (threadIdx.x == 0 && cnt) tile_counts[g] = 0u;
 
-    const unsigned long long shaded = wave_sum_u64(shade_tile<TW, TH>(fr, X0, Y0, key, textures));
-    const unsigned long long cov = wave_sum_u64(covered);
+    const unsigned long long shaded = shade_tile<TW, TH>(fr, X0, Y0, key, textures);
+    const unsigned long long cov = wave_total(covered, lane);
     if (lane == 0) {
         red[wave][0] = cov;
         red[wave][1] = shaded;

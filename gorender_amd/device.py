@@ -170,6 +170,26 @@ class Context:
         _lib.check(self.lib.gr_draw_device_noalloc(self.handle, ctypes.byref(fr), objs,
                                                    len(s.objects), d_rgba, d_depth))
 
+    def present_draw(self, s: SceneData, mesh_handles: Sequence[int],
+                     with_depth: bool = True) -> None:
+        """gr_present_draw: queue a frame whose pixels land in pinned host memory."""
+        fr = frame_struct(s.frame)
+        objs = self.objects_struct(s, mesh_handles)
+        _lib.check(self.lib.gr_present_draw(self.handle, ctypes.byref(fr), objs, len(s.objects),
+                                            1 if with_depth else 0))
+
+    def present_wait(self, width: int, height: int):
+        """gr_present_wait: (rgba, depth or None, stats) of the oldest queued
+        frame, copied out of the context's pinned buffers."""
+        rp = ctypes.POINTER(ctypes.c_uint8)()
+        dp = ctypes.POINTER(ctypes.c_float)()
+        st = _lib.gr_stats()
+        _lib.check(self.lib.gr_present_wait(self.handle, ctypes.byref(rp), ctypes.byref(dp),
+                                            ctypes.byref(st)))
+        rgba = np.ctypeslib.as_array(rp, (height, width, 4)).copy()
+        depth = np.ctypeslib.as_array(dp, (height, width)).copy() if dp else None
+        return rgba, depth, st.as_dict()
+
     def prepare_batch(self, scenes: Sequence[SceneData], mesh_handles: Sequence[Sequence[int]],
                       outputs: Sequence[Tuple[int, int]]):
         """Build the ctypes arguments of a gr_draw_batch call once (for loops

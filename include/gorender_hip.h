@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GR_ABI_VERSION 3
+#define GR_ABI_VERSION 4
 
 typedef enum {
     GR_OK = 0,
@@ -213,6 +213,11 @@ void gr_matrix_screen(int32_t width, int32_t height, float *out);         /* mat
 void gr_matrix_mvp(const float *perspective, const float *view, const float *world,
                    float *out);                                           /* renderer.go:259-262 */
 void gr_vec3_normalize(const float in[3], float out[3]);                  /* vector.go:92-94 */
+/* Frustum.BoxVisibility of an object's 8 bounding-box corners (bbox [8][4],
+ * as NewMesh's boundingBox gives them) transformed by mvp: 0 outside, 1
+ * intersect, 2 inside (renderer.go:268-275, clipping.go:131-154). Every draw
+ * classifies each object this way on the host; -1 for a null argument. */
+int32_t gr_box_visibility(const float *mvp, const float *bbox, float znear, float zfar);
 
 /* ---- drawing ---- */
 /* Synchronous Draw into caller-owned host buffers: rgba [H][W][4], depth [H][W]
@@ -289,6 +294,19 @@ gr_status gr_autotune(gr_context *ctx, const gr_batch_frame *frames, int32_t num
 /* Wait for queued draws; returns GR_ERR_CAPACITY if one of them overflowed its
  * bins (its output is then incomplete and must be drawn again). */
 gr_status gr_synchronize(gr_context *ctx);
+/* The present path of the reference's window loop (main.go:201-227 with
+ * FrameBuffer.SwapBuffers, rasterizer.go:32-34): gr_present_draw queues a
+ * Draw and the copy of its colour (and depth when with_depth) to pinned host
+ * memory the context owns, and returns at once; gr_present_wait waits for the
+ * oldest queued frame and points rgba/depth (depth NULL without with_depth) at
+ * its pixels, which stay valid until the second gr_present_draw after it. Two
+ * frames are in flight at most (Pixels / Pixels2): the draw of frame N+1
+ * overlaps the copy-out of frame N. A frame whose tile lists overflow is drawn
+ * again inside gr_present_wait. */
+gr_status gr_present_draw(gr_context *ctx, const gr_frame *frame, const gr_object *objects,
+                          int32_t num_objects, int32_t with_depth);
+gr_status gr_present_wait(gr_context *ctx, const uint8_t **rgba, const float **depth,
+                          gr_stats *stats);
 /* Stats of the frames of the last completed draw call (after gr_synchronize). */
 gr_status gr_last_stats(gr_context *ctx, gr_stats *stats, int32_t max_frames,
                         int32_t *num_frames);

@@ -12,7 +12,8 @@ host matrix helpers, against what the mathematics fixes:
 * Whole frames (renderer.go:443-474): 16x16 scenes whose every pixel, depth
   and counter is derived by hand (tests/scene_cases.py hand_frame).
 * gr_matrix_* of the product library against the oracle's builders, bit for
-  bit, and against closed forms.
+  bit, and against closed forms; gr_box_visibility (the per-object box test
+  every draw runs on the host) against the oracle's pinned rc_box_visibility.
 """
 import ctypes
 import math
@@ -314,3 +315,32 @@ def test_host_matrices_closed_forms():
             _p(_v3([0, 0, 0])))
     assert abs(r[0, 0]) < 1e-7 and abs(r[1, 1]) < 1e-7
     assert abs(abs(r[0, 1]) - 1) < 1e-7 and abs(abs(r[1, 0]) - 1) < 1e-7
+
+
+def test_host_box_visibility_equals_oracle(lib):
+    """gr_box_visibility = rc_transform_batch of the 8 corners then
+    rc_box_visibility (pinned by hand-classified boxes), on random objects
+    around the frustum's planes, near planes at 0 and beyond."""
+    g = _lib.load()
+    rng = np.random.default_rng(7)
+    seen = set()
+    for k in range(4000):
+        lo = rng.uniform(-3, 3, 3)
+        hi = lo + rng.uniform(0.01, 3, 3)
+        bbox = np.array([[x, y, z, 1.0] for x in (lo[0], hi[0]) for y in (lo[1], hi[1])
+                         for z in (lo[2], hi[2])], F32)
+        fov = float(F32(rng.uniform(0.3, 2.0)))
+        zn = float(F32(0.0 if k % 2 else rng.uniform(0.05, 1.0)))
+        zf = float(F32(rng.uniform(2.0, 60.0)))
+        P = refcpu.matrix_np(lib.rc_perspective_matrix(fov, float(F32(1.5)), zn, zf))
+        V = refcpu.matrix_np(lib.rc_view_matrix(rc_vec3(*rng.uniform(-4, 4, 3).astype(F32)),
+                                                rc_vec3(*rng.uniform(-1, 1, 3).astype(F32)),
+                                                rc_vec3(0, 1, 0)))
+        mvp = np.ascontiguousarray((P.astype(np.float64) @ V).astype(F32))
+        corners = np.ascontiguousarray(bbox.copy())
+        lib.rc_transform_batch(ctypes.byref(refcpu.matrix(mvp)), _p(corners), 8)
+        want = lib.rc_box_visibility(zn, zf, _p(corners))
+        got = g.gr_box_visibility(_p(mvp), _p(bbox), zn, zf)
+        assert got == want, (k, got, want)
+        seen.add(want)
+    assert seen == {0, 1, 2}
