@@ -22,6 +22,7 @@ GR_ERR_UNSUPPORTED = 6
 GR_ERR_CAPACITY = 7
 GR_ERR_CLIP_OVERFLOW = 8
 GR_ERR_TIMEOUT = 9
+GR_ERR_BUSY = 10
 
 GR_SHOW_FACES = 1 << 0
 GR_BACKFACE_CULLING = 1 << 1
@@ -95,7 +96,8 @@ class gr_tune_result(Structure):
 
 class gr_server_config(Structure):
     _fields_ = [("target_budget", c_int64), ("max_batch", c_int32), ("max_inflight", c_int32),
-                ("colour_only", c_int32), ("reserved", c_int32)]
+                ("colour_only", c_int32), ("max_queue", c_int32), ("deadline_ms", c_int32),
+                ("reserved", c_int32)]
 
 
 class gr_serve_result(Structure):

@@ -39,11 +39,13 @@ int64_t now_ns() {
         .count();
 }
 
-// Colour + depth of one request: device target and pinned host copy.
+// Colour + depth of one request: device target (W*H*8 bytes) and pinned host
+// copy (W*H*8 bytes, or W*H*4 when the server reads back colour only).
 struct Target {
     void* dev = nullptr;
     void* host = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;       // device bytes (what the budget counts)
+    size_t host_bytes = 0;
 };
 
 struct Request {
@@ -92,7 +94,7 @@ struct gr_server {
 
     ~gr_server();
     void run();
-    gr_status take_target(size_t bytes, Target* t);
+    gr_status take_target(size_t bytes, size_t host_bytes, Target* t);
     gr_status launch(std::vector<Request*>& reqs);
     gr_status reap(bool wait);
     void finish(Request* r, gr_status st, const char* msg = nullptr);
@@ -105,8 +107,9 @@ size_t target_bytes(const gr_frame& f) { return (size_t)f.width * f.height * 8; 
 }  // namespace
 
 // Caller holds m.
-gr_status gr_server::take_target(size_t bytes, Target* t) {
+gr_status gr_server::take_target(size_t bytes, size_t host_bytes, Target* t) {
     t->bytes = bytes;
+    t->host_bytes = host_bytes;
     auto d = free_dev.find(bytes);
     if (d != free_dev.end()) {
         t->dev = d->second;
@@ -115,11 +118,11 @@ gr_status gr_server::take_target(size_t bytes, Target* t) {
         t->dev = nullptr;
         return fail(GR_ERR_OUT_OF_MEMORY, "device target");
     }
-    auto h = free_host.find(bytes);
+    auto h = free_host.find(host_bytes);
     if (h != free_host.end()) {
         t->host = h->second;
         free_host.erase(h);
-    } else if (hipHostMalloc(&t->host, bytes, hipHostMallocDefault) != hipSuccess) {
+    } else if (hipHostMalloc(&t->host, host_bytes, hipHostMallocDefault) != hipSuccess) {
         free_dev.emplace(bytes, t->dev);
         t->dev = t->host = nullptr;
         return fail(GR_ERR_OUT_OF_MEMORY, "pinned host target");
@@ -149,8 +152,11 @@ gr_status gr_server::launch(std::vector<Request*>& reqs) {
                           gr_internal::stats_bytes_per_frame() * (size_t)cfg.max_batch,
                           hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void**)&b.h_ctrl, gr_internal::control_bytes(), hipHostMallocDefault) !=
-                hipSuccess)
+                hipSuccess) {
+            // The parts that were created go to the spare list (freed with the server).
+            spare.push_back(b);
             return fail(GR_ERR_DEVICE, "server batch resources");
+        }
     }
     std::vector<gr_batch_frame> frames(reqs.size());
     std::vector<gr_object> objs;
@@ -182,19 +188,23 @@ gr_status gr_server::launch(std::vector<Request*>& reqs) {
         hipMemsetAsync(gr_internal::control_device(ctx), 0, gr_internal::control_bytes(), s) !=
             hipSuccess ||
         hipEventRecord(b.rendered, s) != hipSuccess ||
-        hipStreamWaitEvent(copy_stream, b.rendered, 0) != hipSuccess)
+        hipStreamWaitEvent(copy_stream, b.rendered, 0) != hipSuccess) {
+        spare.push_back(b);
         return fail(GR_ERR_DEVICE, "server batch readback");
+    }
     // Frames to pinned host memory on the copy stream, overlapping the next
     // batch: colour, then depth unless the server reads back colour only.
     for (Request* r : reqs) {
-        const size_t n = cfg.colour_only ? (size_t)r->frame.width * r->frame.height * 4
-                                         : r->target.bytes;
-        if (hipMemcpyAsync(r->target.host, r->target.dev, n, hipMemcpyDeviceToHost,
-                           copy_stream) != hipSuccess)
+        if (hipMemcpyAsync(r->target.host, r->target.dev, r->target.host_bytes,
+                           hipMemcpyDeviceToHost, copy_stream) != hipSuccess) {
+            spare.push_back(b);
             return fail(GR_ERR_DEVICE, "server frame readback");
+        }
     }
-    if (hipEventRecord(b.copied, copy_stream) != hipSuccess)
+    if (hipEventRecord(b.copied, copy_stream) != hipSuccess) {
+        spare.push_back(b);
         return fail(GR_ERR_DEVICE, "server batch event");
+    }
     for (Request* r : reqs) {
         r->result.start_ns = b.start_ns;
         r->result.batch_size = (int32_t)reqs.size();
@@ -272,15 +282,25 @@ void gr_server::run() {
             // Admission: in arrival order, while the target budget and the
             // batch and in-flight limits allow (a request that does not fit
             // waits for earlier ones to complete).
+            const int64_t now = now_ns();
             while (!queue.empty() && (int)batch.size() < cfg.max_batch &&
                    (int)inflight.size() < cfg.max_inflight) {
                 Request* r = queue.front();
+                // Shed a request that waited past the deadline before it
+                // could be admitted (a redraw after an overflow is kept).
+                if (cfg.deadline_ms > 0 && !r->target.dev &&
+                    now - r->result.submit_ns > (int64_t)cfg.deadline_ms * 1000000) {
+                    queue.pop_front();
+                    finish(r, GR_ERR_BUSY, "shed: waited past the deadline before admission");
+                    cv_done.notify_all();
+                    continue;
+                }
                 const size_t need = r->target.dev ? 0 : target_bytes(r->frame);
                 if (need && used + need > (size_t)cfg.target_budget) break;
                 queue.pop_front();
                 if (need) {
                     Target t;
-                    gr_status ts = take_target(need, &t);
+                    gr_status ts = take_target(need, cfg.colour_only ? need / 2 : need, &t);
                     if (ts) {
                         finish(r, ts);
                         cv_done.notify_all();
@@ -364,6 +384,8 @@ gr_status gr_server_create(gr_context* ctx, const gr_server_config* cfg, gr_serv
         return fail(GR_ERR_INVALID_ARGUMENT, "server budget, max_batch and max_inflight must be positive");
     if ((cfg->colour_only != 0 && cfg->colour_only != 1) || cfg->reserved != 0)
         return fail(GR_ERR_INVALID_ARGUMENT, "colour_only must be 0 or 1 and reserved 0");
+    if (cfg->max_queue < 0 || cfg->deadline_ms < 0)
+        return fail(GR_ERR_INVALID_ARGUMENT, "max_queue and deadline_ms must not be negative");
     try {
         std::unique_ptr<gr_server> s(new gr_server);
         s->ctx = ctx;
@@ -372,7 +394,9 @@ gr_status gr_server_create(gr_context* ctx, const gr_server_config* cfg, gr_serv
             hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
             return fail(GR_ERR_DEVICE, "server copy stream");
         gr_server* raw = s.get();
-        gr_internal::set_one_pipe(ctx, true);
+        // Consecutive batches use the context's pipelines, so a batch's
+        // geometry overlaps the previous one's raster; each batch's stats and
+        // control words are read on its own pipe's stream.
         s->worker = std::thread([raw] { raw->run(); });
         *out = s.release();
         return GR_OK;
@@ -389,7 +413,6 @@ void gr_server_destroy(gr_server* s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    gr_internal::set_one_pipe(s->ctx, false);
     delete s;
 }
 
@@ -412,6 +435,8 @@ gr_status gr_server_submit(gr_server* s, const gr_frame* frame, const gr_object*
         std::lock_guard<std::mutex> lk(s->m);
         if (s->stop || s->fatal) return fail(s->fatal ? s->fatal : GR_ERR_INVALID_ARGUMENT,
                                              "server stopped");
+        if (s->cfg.max_queue > 0 && (int64_t)s->queue.size() >= s->cfg.max_queue)
+            return fail(GR_ERR_BUSY, "server queue full");
         r->ticket = s->next_ticket++;
         r->result.submit_ns = now_ns();
         *ticket = r->ticket;
@@ -447,9 +472,15 @@ gr_status gr_server_release(gr_server* s, int64_t ticket) {
     if (it == s->all.end()) return fail(GR_ERR_INVALID_HANDLE, "unknown ticket");
     Request* r = it->second.get();
     if (!r->done) return fail(GR_ERR_INVALID_ARGUMENT, "request still in progress");
-    // The pinned half goes back to the pool (the worker reuses it by size).
-    if (r->target.host) s->free_host.emplace(r->target.bytes, r->target.host);
+    // The pinned half goes back to the pool (the worker reuses it by size); a
+    // request that failed still holding its device half returns that too.
+    if (r->target.host) s->free_host.emplace(r->target.host_bytes, r->target.host);
     r->target.host = nullptr;
+    if (r->target.dev) {
+        s->free_dev.emplace(r->target.bytes, r->target.dev);
+        s->used -= r->target.bytes;
+        r->target.dev = nullptr;
+    }
     s->all.erase(it);
     return GR_OK;
 }
@@ -480,12 +511,17 @@ gr_status gr_server_replay(gr_server* s, const gr_frame* frames, const gr_object
                 if (submitted <= i) return;
                 t = tickets[(size_t)i];
             }
+            if (t == 0) {  // refused at submission (queue full)
+                if (latency_ns) latency_ns[i] = -1;
+                if (batch_size) batch_size[i] = 0;
+                continue;
+            }
             gr_serve_result r;
             const gr_status st = gr_server_wait(s, t, -1, &r);
-            if (latency_ns) latency_ns[i] = r.done_ns - r.submit_ns;
+            if (latency_ns) latency_ns[i] = st == GR_ERR_BUSY ? -2 : r.done_ns - r.submit_ns;
             if (batch_size) batch_size[i] = r.batch_size;
             (void)gr_server_release(s, t);
-            if (st) {
+            if (st && st != GR_ERR_BUSY) {
                 std::lock_guard<std::mutex> lk(tm);
                 if (!first_error) first_error = st;
             }
@@ -502,6 +538,10 @@ gr_status gr_server_replay(gr_server* s, const gr_frame* frames, const gr_object
         int64_t t;
         st = gr_server_submit(s, &frames[d], objects ? objects + first_object[d] : nullptr,
                               num_objects[d], &t);
+        if (st == GR_ERR_BUSY) {  // a full queue refuses it: counted, not fatal
+            t = 0;
+            st = GR_OK;
+        }
         std::lock_guard<std::mutex> lk(tm);
         if (st) {
             aborted = true;

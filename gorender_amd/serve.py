@@ -45,10 +45,12 @@ class PreparedRequest:
 
 class Server:
     def __init__(self, ctx: Context, target_budget: int = 1 << 30, max_batch: int = 16,
-                 max_inflight: int = 2, colour_only: bool = False):
+                 max_inflight: int = 2, colour_only: bool = False, max_queue: int = 0,
+                 deadline_ms: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
-        cfg = _lib.gr_server_config(target_budget, max_batch, max_inflight, int(colour_only), 0)
+        cfg = _lib.gr_server_config(target_budget, max_batch, max_inflight, int(colour_only),
+                                    max_queue, deadline_ms, 0)
         h = ctypes.c_void_p()
         _lib.check(self.lib.gr_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
@@ -110,7 +112,8 @@ def poisson_arrivals(n: int, qps: float, seed: int) -> np.ndarray:
 def replay(server: Server, requests: Sequence[PreparedRequest], arrivals: np.ndarray):
     """Open-loop load (gr_server_replay): request i % len(requests) is
     submitted at arrivals[i] seconds; returns (latency_ms, batch_size) arrays,
-    latency from submission to colour and depth in host memory."""
+    latency from submission to colour and depth in host memory (-1e-6 ms for a
+    request a full queue refused, -2e-6 ms for one shed past the deadline)."""
     d = len(requests)
     frames = (_lib.gr_frame * d)()
     total = sum(r.num_objects for r in requests)

@@ -57,6 +57,9 @@ typedef enum {
     GR_ERR_CLIP_OVERFLOW = 8,
     /* gr_server_wait: the request did not finish within the timeout. */
     GR_ERR_TIMEOUT = 9,
+    /* Serving: the queue was full at submission, or the request waited past
+     * the deadline before admission and was shed. */
+    GR_ERR_BUSY = 10,
 } gr_status;
 
 /* Renderer toggles (renderer.go:90-97). Defaults of NewRenderer
@@ -382,6 +385,10 @@ typedef struct {
     int32_t max_inflight;  /* launches queued on the device at once */
     int32_t colour_only;   /* 1: read back colour only (result depth = NULL), as a
                               window's present needs (main.go:296-298); 0: both */
+    int32_t max_queue;     /* requests waiting for admission at most (0: no bound);
+                              gr_server_submit beyond it -> GR_ERR_BUSY */
+    int32_t deadline_ms;   /* a request that waited longer for admission is shed:
+                              its wait returns GR_ERR_BUSY (0: never) */
     int32_t reserved;      /* 0 */
 } gr_server_config;
 typedef struct {
@@ -417,7 +424,8 @@ gr_status gr_server_release(gr_server *server, int64_t ticket);
  * with objects[first_object[d] .. + num_objects[d]]) at arrival_s[i] seconds
  * after the call starts (busy-waiting on the clock), while a second thread
  * waits for and releases results in submission order; latency_ns[i]
- * (submit -> colour and depth in host memory) and batch_size[i] may be NULL. */
+ * (submit -> colour and depth in host memory; -1 refused by a full queue, -2
+ * shed past the deadline) and batch_size[i] may be NULL. */
 gr_status gr_server_replay(gr_server *server, const gr_frame *frames, const gr_object *objects,
                            const int32_t *first_object, const int32_t *num_objects,
                            int32_t num_distinct, const double *arrival_s, int32_t n,

@@ -186,7 +186,47 @@ def test_colour_only(ctx, resident):
 
 
 def test_server_config_checked(ctx):
-    bad = _lib.gr_server_config(1 << 20, 4, 2, 2, 0)
     h = ctypes.c_void_p()
-    assert ctx.lib.gr_server_create(ctx.handle, ctypes.byref(bad), ctypes.byref(h)) == \
-        _lib.GR_ERR_INVALID_ARGUMENT
+    for bad in (_lib.gr_server_config(1 << 20, 4, 2, 2, 0, 0, 0),
+                _lib.gr_server_config(1 << 20, 4, 2, 0, -1, 0, 0),
+                _lib.gr_server_config(1 << 20, 4, 2, 0, 0, -5, 0)):
+        assert ctx.lib.gr_server_create(ctx.handle, ctypes.byref(bad), ctypes.byref(h)) == \
+            _lib.GR_ERR_INVALID_ARGUMENT
+
+
+def test_bounded_queue_refuses_and_admitted_are_exact(ctx, resident):
+    """max_queue: a burst beyond the queue bound is refused at submission
+    (GR_ERR_BUSY); every admitted request is still exact."""
+    rs, reqs = resident
+    tickets, refused = [], 0
+    with Server(ctx, 1 << 30, 8, 2, max_queue=8) as srv:
+        for i in range(256):
+            try:
+                tickets.append((i % 8, srv.submit(reqs[i % 8])))
+            except _lib.GorenderError as e:
+                assert e.status == _lib.GR_ERR_BUSY
+                refused += 1
+        results = [(k, srv.wait(t)) for k, t in tickets]
+    print(f"{len(results)} admitted, {refused} refused", flush=True)
+    assert refused > 0 and results
+    for k, res in results:
+        check_result(k, res)
+
+
+def test_deadline_sheds_late_requests(ctx, resident):
+    """deadline_ms: requests that waited longer for admission are shed (their
+    wait returns GR_ERR_BUSY); the others are exact."""
+    rs, reqs = resident
+    shed, done = 0, 0
+    with Server(ctx, 1 << 30, 2, 1, deadline_ms=1) as srv:
+        tickets = [(i % 8, srv.submit(reqs[i % 8])) for i in range(64)]
+        for k, t in tickets:
+            try:
+                check_result(k, srv.wait(t))
+                done += 1
+            except _lib.GorenderError as e:
+                assert e.status == _lib.GR_ERR_BUSY, e
+                srv.release(t)
+                shed += 1
+    print(f"{done} served, {shed} shed", flush=True)
+    assert shed > 0 and done > 0
