@@ -11,6 +11,9 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int32, c_
                     c_uint8, c_uint32, c_void_p)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgorender_hip.so")
+# Another build of the library can be named here (the sanitizer test loads the
+# host-only ASan/UBSan build this way); symbols it lacks are left unbound.
+ALT_LIB_PATH = os.environ.get("GORENDER_HIP_LIB")
 
 GR_OK = 0
 GR_ERR_INVALID_ARGUMENT = 1
@@ -203,10 +206,13 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
-    lib = ctypes.CDLL(LIB_PATH)
+    path = ALT_LIB_PATH or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
+        if ALT_LIB_PATH and not hasattr(lib, name):
+            continue
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args

@@ -13,7 +13,7 @@ from ctypes import POINTER, Structure, c_float, c_int32, c_int64, c_uint8, c_uin
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "build", "librefcpu.so")
+LIB_PATH = os.environ.get("REFCPU_LIB") or os.path.join(HERE, "build", "librefcpu.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libmatrix_amd64.so")
 
 
