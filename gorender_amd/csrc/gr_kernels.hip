@@ -1572,6 +1572,36 @@ constexpr int raster_waves(int npx) {
                : (160 * 1024) / (npx * 8 + kRasterLdsFixed);
 }
 
+// Walk form of a record for one tile (piece 2 of the staged record): the box
+// clamped to the tile as one word (x0 - X0 | y0 - Y0 << 6 | x1 - x0 << 12 |
+// height << 18 | log2 lanes per row << 25 | wide << 28; height 0 when the box
+// misses the tile) and, on the int32 path, the three edge values at the
+// clamped box's corner (x0, y0), A*x0 + B*y0 + C (rasterizer.go:107-129 at the
+// first pixel of the walk). Inputs are the record's pieces 2..4: box x, box y,
+// wide, A01 | B01, C01, A12, B12 | C12, A20, B20, C20.
+constexpr uint32_t kWalkWide = 1u << 28;
+template <int TW, int TH>
+__device__ __forceinline__ uint4 walk_form(const uint4& w2, const uint4& w3, const uint4& w4,
+                                           int X0, int Y0) {
+    const int x0 = max((int)(int16_t)(w2.x & 0xFFFFu), X0);
+    const int x1 = min((int)(int16_t)(w2.x >> 16), X0 + TW - 1);
+    const int y0 = max((int)(int16_t)(w2.y & 0xFFFFu), Y0);
+    const int y1 = min((int)(int16_t)(w2.y >> 16), Y0 + TH - 1);
+    if (x1 < x0 || y1 < y0) return make_uint4(0u, 0u, 0u, 0u);
+    const int bw = x1 - x0 + 1;
+    int l2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
+    if (l2 > 6) l2 = 6;
+    uint32_t info = (uint32_t)(x0 - X0) | (uint32_t)(y0 - Y0) << 6 | (uint32_t)(x1 - x0) << 12 |
+                    (uint32_t)(y1 - y0 + 1) << 18 | (uint32_t)l2 << 25;
+    if (w2.z) return make_uint4(info | kWalkWide, 0u, 0u, 0u);
+    // |A|, |B| <= 2^15 and |x|, |y| < 2^14 on the int32 path: exact 24-bit
+    // products, and the sums fit int32.
+    const int f0 = __mul24((int)w2.w, x0) + __mul24((int)w3.x, y0) + (int)w3.y;
+    const int f1 = __mul24((int)w3.z, x0) + __mul24((int)w3.w, y0) + (int)w4.x;
+    const int f2 = __mul24((int)w4.y, x0) + __mul24((int)w4.z, y0) + (int)w4.w;
+    return make_uint4(info, (uint32_t)f0, (uint32_t)f1, (uint32_t)f2);
+}
+
 // Resolves n tile-list entries (slots) into the tile's depth keys: the
 // visibility part of rasterizer.go:139-156 for every listed triangle. Returns
 // this lane's share of the covered-fragment count (summed over the wave).
@@ -1583,19 +1613,33 @@ __device__ __forceinline__ unsigned long long raster_list(
     constexpr int CHUNK = kRasterChunk;  // triangles staged in LDS at a time
     static_assert(TW * TH <= 4096 && CHUNK <= (1 << 20),
                   "fragment codes hold the pixel index in 12 bits");
+    static_assert(TW <= 64 && TH <= 64, "a clamped box fits one column block of lanes");
     unsigned long long covered_lane = 0;  // wide path, per lane
     uint32_t covered_wave = 0;            // int32 path, wave-uniform
     for (uint32_t base = 0; base < cnt; base += CHUNK) {
         const uint32_t n = min(cnt - base, (uint32_t)CHUNK);
-        // Stage the chunk: slots (one coalesced read), then the raster records
-        // as 16-byte pieces, all loads of the chunk in flight together.
+        // Stage the chunk: slots (one coalesced read), then the raster records,
+        // all loads of the chunk in flight together. Waves 1 and 2 copy the
+        // divisors and reciprocals (pieces 0 and 1) of record t; wave 0 reads
+        // the box and edge coefficients of record t and writes their walk
+        // form in place of pieces 2..4 (once per record, a lane each, rather
+        // than once per record in every wave's scalar unit).
         __syncthreads();
         if (threadIdx.x < n) lslot[threadIdx.x] = list[base + threadIdx.x];
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < 5 * n; p += 256) {
-            const uint32_t ri = p / 5u, pi = p - ri * 5u;
-            const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[ri]]);
-            lrec[p] = gload(src + pi);
+        {
+            const uint32_t t = threadIdx.x & 63u;
+            if (wave < 3 && t < n) {
+                const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[t]]);
+                if (wave > 0) {
+                    lrec[t * 5 + wave - 1] = gload(src + wave - 1);
+                } else {
+                    const uint4 w2 = gload(src + 2), w3 = gload(src + 3), w4 = gload(src + 4);
+                    lrec[t * 5 + 2] = walk_form<TW, TH>(w2, w3, w4, X0, Y0);
+                    lrec[t * 5 + 3] = make_uint4(w2.w, w3.z, w4.y, w3.x);  // A01 A12 A20 B01
+                    lrec[t * 5 + 4] = make_uint4(w3.w, w4.z, 0u, 0u);      // B12 B20
+                }
+            }
         }
         __syncthreads();
         // Per wave, one triangle at a time: lanes cover an lpr x (64/lpr) pixel
@@ -1604,21 +1648,19 @@ __device__ __forceinline__ unsigned long long raster_list(
         // to an LDS ring by ballot and their exact depth runs on full waves.
         uint32_t qhead = 0, qtail = 0;  // wave-uniform ring positions
         for (uint32_t i = wave; i < n; i += 4) {
-            // The record's box and edge coefficients are wave-uniform: read
-            // from LDS and moved to scalar registers.
+            // The walk form of the record (see walk_form): the box word moves
+            // to a scalar register, the coefficients stay in vector registers
+            // (the same value in every lane).
             const uint4 w2 = lrec[i * 5 + 2], w3 = lrec[i * 5 + 3], w4 = lrec[i * 5 + 4];
-            const uint32_t bxw = (uint32_t)sgpr((int)w2.x);
-            const uint32_t byw = (uint32_t)sgpr((int)w2.y);
-            const int x0 = max((int)(int16_t)(bxw & 0xFFFFu), X0);
-            const int x1 = min((int)(int16_t)(bxw >> 16), X0 + TW - 1);
-            const int y0 = max((int)(int16_t)(byw & 0xFFFFu), Y0);
-            const int y1 = min((int)(int16_t)(byw >> 16), Y0 + TH - 1);
-            const int bw = x1 - x0 + 1;
-            int l2 = bw <= 1 ? 0 : 32 - __builtin_clz((uint32_t)(bw - 1));
-            if (l2 > 6) l2 = 6;
+            const uint32_t info = (uint32_t)sgpr((int)w2.x);
+            const int bh = (int)((info >> 18) & 127u);
+            if (bh == 0) continue;  // the box misses the tile
+            const int ox = (int)(info & 63u), oy = (int)((info >> 6) & 63u);
+            const int dxm = (int)((info >> 12) & 63u), l2 = (int)((info >> 25) & 7u);
+            const int x0 = X0 + ox, x1 = x0 + dxm, y0 = Y0 + oy, y1 = y0 + bh - 1;
             const int lpr = 1 << l2, rows = 64 >> l2;
             const int lx = lane & (lpr - 1), ly = lane >> l2;
-            if (sgpr((int)w2.z)) {
+            if (info & kWalkWide) {
                 // Coordinates beyond the int32 range: direct int64 evaluation.
                 const uint32_t slot = (uint32_t)sgpr((int)lslot[i]);
                 const WideRec* wr = &fr.wrec[slot];
@@ -1645,21 +1687,22 @@ __device__ __forceinline__ unsigned long long raster_list(
                 }
                 continue;
             }
-            const int A0 = sgpr((int)w2.w), B0 = sgpr((int)w3.x), C0 = sgpr((int)w3.y);
-            const int A1 = sgpr((int)w3.z), B1 = sgpr((int)w3.w), C1 = sgpr((int)w4.x);
-            const int A2 = sgpr((int)w4.y), B2 = sgpr((int)w4.z), C2 = sgpr((int)w4.w);
-            // |A|, |B| <= 2^15 and |x|, |y| < 2^14 on this path: 24-bit products
-            // are exact and every edge value fits int32.
-            const int px = x0 + lx, py = y0 + ly;
-            int f0 = __mul24(A0, px) + __mul24(B0, py) + C0;
-            int f1 = __mul24(A1, px) + __mul24(B1, py) + C1;
-            int f2 = __mul24(A2, px) + __mul24(B2, py) + C2;
-            const int sx0 = A0 * lpr, sx1 = A1 * lpr, sx2 = A2 * lpr;
-            const int sy0 = B0 * rows, sy1 = B1 * rows, sy2 = B2 * rows;
+            // Edge values of the lane's pixel: the value at the clamped box's
+            // corner (x0, y0) plus A*lx + B*ly; |A|, |B| <= 2^15 and lx, ly
+            // < 64 on this path, so the 24-bit products are exact and every
+            // edge value fits int32 (|x|, |y| < 2^14).
+            const int A0 = (int)w3.x, A1 = (int)w3.y, A2 = (int)w3.z;
+            const int B0 = (int)w3.w, B1 = (int)w4.x, B2 = (int)w4.y;
+            int f0 = __mul24(A0, lx) + __mul24(B0, ly) + (int)w2.y;
+            int f1 = __mul24(A1, lx) + __mul24(B1, ly) + (int)w2.z;
+            int f2 = __mul24(A2, lx) + __mul24(B2, ly) + (int)w2.w;
+            const int sy0 = (int)((uint32_t)B0 << (6 - l2));  // B * rows
+            const int sy1 = (int)((uint32_t)B1 << (6 - l2));
+            const int sy2 = (int)((uint32_t)B2 << (6 - l2));
             // Fragment code: list index << 12 | the pixel's index in the tile
             // (y * TW + x, below 4096 for every tile size), exact in every lane
             // that enqueues (its pixel lies inside the tile).
-            uint32_t prow = (i << 12) + (uint32_t)(py - Y0) * (uint32_t)TW + (uint32_t)(px - X0);
+            const uint32_t prow = (i << 12) + (uint32_t)(oy * TW + ox) + (uint32_t)(ly * TW + lx);
             // One walk step: lanes with all < 0 hold an inside fragment (every
             // edge value < 0, rasterizer.go:147, and the pixel inside the box);
             // they go to the wave's ring by ballot, and every 64 queued
@@ -1686,44 +1729,24 @@ __device__ __forceinline__ unsigned long long raster_list(
                     }
                 }
             };
-            if (x1 - x0 < lpr) {
-                // The box is one column block wide (the common case): a flat
-                // loop over row blocks. The lane's column is inside the box
-                // iff gx < 0; only the last block can hold rows beyond it.
-                const int gx = lx - (x1 - x0) - 1;
-                const int nfull = (y1 - y0 + 1) >> (6 - l2);  // blocks of `rows` whole rows
-                uint32_t pk = prow;
-                for (int sb = 0; sb < nfull; sb++) {
-                    step(f0 & f1 & f2 & gx, f1, f2, pk);
-                    f0 += sy0;
-                    f1 += sy1;
-                    f2 += sy2;
-                    pk += (uint32_t)rows * (uint32_t)TW;
-                }
-                const int rest = (y1 - y0 + 1) - (nfull << (6 - l2));
-                if (rest > 0) {
-                    const int gy = ly - rest;  // negative iff the lane's row is inside
-                    step(f0 & f1 & f2 & gx & gy, f1, f2, pk);
-                }
-                continue;
-            }
-            for (int yb = y0; yb <= y1; yb += rows) {
-                // Negative exactly when the lane's row is inside the box.
-                const int gy = ly - (y1 - yb) - 1;
-                int g0 = f0, g1 = f1, g2 = f2;
-                uint32_t pk = prow;
-                for (int xb = x0; xb <= x1; xb += lpr) {
-                    const int gx = lx - (x1 - xb) - 1;
-                    step(g0 & g1 & g2 & gx & gy, g1, g2, pk);
-                    g0 += sx0;
-                    g1 += sx1;
-                    g2 += sx2;
-                    pk += (uint32_t)lpr;
-                }
+            // The box is at most TW <= 64 pixels wide, so one column block of
+            // lanes covers it: a flat loop over row blocks. The lane's column
+            // is inside the box iff gx < 0; only the last block can hold rows
+            // beyond it.
+            const int gx = lx - dxm - 1;
+            const int nfull = bh >> (6 - l2);  // blocks of `rows` whole rows
+            uint32_t pk = prow;
+            for (int sb = 0; sb < nfull; sb++) {
+                step(f0 & f1 & f2 & gx, f1, f2, pk);
                 f0 += sy0;
                 f1 += sy1;
                 f2 += sy2;
-                prow += (uint32_t)rows * (uint32_t)TW;
+                pk += (uint32_t)rows * (uint32_t)TW;
+            }
+            const int rest = bh - (nfull << (6 - l2));
+            if (rest > 0) {
+                const int gy = ly - rest;  // negative iff the lane's row is inside
+                step(f0 & f1 & f2 & gx & gy, f1, f2, pk);
             }
         }
         while (qtail != qhead) {
