@@ -1,18 +1,25 @@
 """Benchmark of the render path on one MI355X.
 
 Workload: the headline frame (displaced lat-long sphere of 200,000 triangles,
-16x16 RGBA8 texture, 1280x720, camera at r=1.2). A step is one frame: the full
-path (vertex transform, clip, bin, raster, shade, colour+depth written to HBM)
-with meshes and textures resident in HBM. Frame k rotates the object by
+16x16 RGBA8 texture, 1280x720, camera at r=1.2). Frame k rotates the object by
 0.01*k rad about Y, as the reference's demo loop does (main.go:229-233), so
-every frame is distinct work.
+the frames of a launch are distinct work. A step is one batched launch of
+--batch frames (64 by default, k = 0..63): the full path (vertex transform,
+clip, bin, raster, shade, colour + depth written to HBM) over 64 distinct
+frames, meshes and textures resident in HBM. --warmup steps run untimed first, then exactly --steps steps
+are timed between two device synchronisations.
 
-Modes timed: "batched" (B frames per gr_draw_batch call, the reported value)
-and "plain" (one gr_draw_device call per frame). Latency p50/p99 is measured
-per frame from submit to output-ready in device memory, and separately to
-output-ready in host memory (PCIe readback included).
+Forms timed: "graph" (each batch's launches captured in HIP graphs and
+replayed, two batches alternating on the context's two pipelines), "stream"
+(the same batches submitted with gr_draw_batch) and "plain" (one
+gr_draw_device call per frame, with the library's two pipelines and strictly
+one after another). The value is the faster batched form. Beside it: config 4
+(256 mixed frames: autotuned configuration, one graph launch for all 256
+against 256 plain launches, outputs compared on the device), configs 3 and 5,
+the serving loop at fixed request rates, per-kernel times, the HBM roofline of
+the dominant kernel and a CPU baseline.
 
-Prints one JSON line (the contract of this repo's driver).
+Prints one JSON line.
 """
 import argparse
 import json
@@ -30,20 +37,21 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4096)
-    ap.add_argument("--warmup", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=64,
+                    help="timed steps; a step is one batched launch of --batch frames")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed steps before timing")
     ap.add_argument("--batch", type=int, default=64, help="frames per batched launch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for rocprofv3: no CPU baseline, no latency loops")
-    ap.add_argument("--mode", choices=["both", "batched", "plain", "config5"], default="both",
+                    help="short run for rocprofv3: no CPU baseline, no latency loops, no "
+                         "other configs")
+    ap.add_argument("--mode", choices=["all", "stream", "graph", "plain", "config5"],
+                    default="all",
                     help="forms to run (rocprofv3 runs profile one form at a time); config5 "
                          "only launches the 8-request serving batch --steps times")
-    ap.add_argument("--tile", default="32x32",
-                    help="raster tile WxH, or 'auto' to time the candidates first and keep "
-                         "the fastest")
+    ap.add_argument("--tile", default="32x32", help="raster tile WxH")
     ap.add_argument("--tile-split", type=int, default=0,
                     help="tile-list entries per raster workgroup before a tile is split "
                          "(0 keeps the library default)")
@@ -51,11 +59,15 @@ def parse():
                     help="context pipelines: 2 overlaps a draw's geometry kernels with the "
                          "previous draw's raster (the library default)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the throughput lines of configs 3-5")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="also run k_copy over this many MiB (known-size kernel that the "
-                         "HBM byte counters are calibrated against)")
+                    help="skip configs 3-5 and the serving loop")
+    ap.add_argument("--calib-copy-mib", type=int, default=1024,
+                    help="MiB copied by k_copy16 to measure the device's copy bandwidth (also "
+                         "the known-size kernel that HBM byte counters are calibrated against)")
     return ap.parse_args()
+
+
+def profile_file(name):
+    return os.path.join(ROOT, "profiles", name)
 
 
 def measured_traffic(kernel, cfg, frames_per_launch):
@@ -63,7 +75,7 @@ def measured_traffic(kernel, cfg, frames_per_launch):
     summary (FETCH_SIZE + WRITE_SIZE, each scaled by the ratio a known-size
     copy kernel reads under the same counters), when it was taken on this
     kernel, tile and batch size; else None."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_config2.txt")
+    path = profile_file("hbm_traffic_config2.txt")
     if not os.path.exists(path):
         return None
     want = f"{kernel} tile={cfg.tile_w}x{cfg.tile_h} frames_per_launch={frames_per_launch}"
@@ -78,32 +90,174 @@ def measured_traffic(kernel, cfg, frames_per_launch):
 VALU_PEAK_GINSTR_S = 256 * 4 * 2.4 / 4
 
 
-def measured_valu(kernel, cfg, frames_per_launch):
-    """VALU wave-instructions per launch of `kernel` (SQ_INSTS_VALU) from the
-    committed rocprofv3 counter summary of the headline batch, when it was
-    taken with this tile and 64 frames per launch; else None."""
-    path = os.path.join(ROOT, "profiles", "sq_counters_config2.txt")
+def sq_counters(kernel, cfg, frames_per_launch):
+    """SQ counters per launch of `kernel` from the committed rocprofv3 summary
+    of the headline batch, when it was taken with this tile and batch size."""
+    path = profile_file("sq_counters_config2.txt")
     if frames_per_launch != 64 or not os.path.exists(path):
-        return None
+        return {}
     name = f"{kernel}<{cfg.tile_w}, {cfg.tile_h}>" if kernel.startswith("k_raster") else kernel
-    tile_ok = False
+    tile_ok, out = False, {}
     for line in open(path):
         if line.startswith("# raster tile "):
             tile_ok = line.split()[-1] == f"{cfg.tile_w}x{cfg.tile_h}"
-        elif tile_ok and line.startswith(name + " ") and "SQ_INSTS_VALU=" in line:
-            return int(line.split("SQ_INSTS_VALU=")[1].split()[0])
-    return None
+        elif tile_ok and line.startswith(name + " "):
+            for kv in line.split()[len(name.split()):]:
+                if "=" in kv:
+                    k, v = kv.split("=", 1)
+                    try:
+                        out[k] = float(v)
+                    except ValueError:  # the [vgpr=.. scratch=..] annotation
+                        pass
+    return out
+
+
+def capacity_safe(ctx, fn):
+    """Runs fn() then waits; a tile-list overflow (the context grows its
+    capacity) runs it again. Returns the passes run."""
+    from gorender_amd import _lib
+    for n in range(1, 9):
+        fn()
+        try:
+            ctx.synchronize()
+            return n
+        except _lib.GorenderError as e:
+            if e.status != _lib.GR_ERR_CAPACITY:
+                raise
+    raise RuntimeError("tile-list capacity did not settle")
+
+
+def warm_until_stable(ctx, fn, min_passes=4):
+    """Untimed passes until two in a row needed no regrowth and at least
+    min_passes ran (with two pipelines both pipes then hold buffers of their
+    final size, so no allocation falls inside a timed region)."""
+    from gorender_amd import _lib
+    n = clean = 0
+    while (clean < 2 or n < min_passes) and n < 32:
+        fn()
+        n += 1
+        try:
+            ctx.synchronize()
+            clean += 1
+        except _lib.GorenderError as e:
+            if e.status != _lib.GR_ERR_CAPACITY:
+                raise
+            clean = 0
+    return n
+
+
+def timed_passes(ctx, fn, min_s=0.25, min_passes=20):
+    """Passes of fn until at least min_s seconds and min_passes passes have
+    run; returns (passes, seconds)."""
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        fn()
+        n += 1
+        if n >= min_passes and n % 4 == 0:
+            ctx.synchronize()
+            dt = time.perf_counter() - t0
+            if dt >= min_s:
+                return n, dt
+
+
+def frames_digest(ctx, outs):
+    """gr_device_checksum over every frame's colour and depth buffers."""
+    import numpy as np
+    d = ctx.alloc(8)
+    try:
+        d.fill(0)
+        for rg, dp in outs:
+            rg.checksum_into(d)
+            dp.checksum_into(d)
+        ctx.synchronize()
+        return int(d.to_numpy(np.uint64, (1,))[0])
+    finally:
+        d.free()
+
+
+def config4_leg(ctx, scenes):
+    """Config 4: 256 independent frames (1k-500k triangles, four resolutions,
+    half textured). The autotuner picks tile size, tile_split and tile-list
+    capacity on the batch; then one graph launch draws all 256 frames, against
+    256 plain gr_draw_device launches (the library's two pipelines, and
+    strictly one after another) under the same configuration. The outputs of
+    the forms are compared by device checksums."""
+    frames = scenes.config4_frames(256)
+    handles = [ctx.upload_scene(s)[0] for s in frames]
+    outs = [ctx.frame_buffers(s.frame.width, s.frame.height) for s in frames]
+    ptrs = [(o[0].ptr, o[1].ptr) for o in outs]
+    tris = sum(s.num_faces for s in frames)
+    saved = ctx.config()
+    res = {"frames": len(frames), "triangles_total": tris}
+    try:
+        prepared = ctx.prepare_batch(frames, handles, ptrs)
+        t0 = time.perf_counter()
+        best, tried = ctx.autotune(prepared, reps=2)
+        res["autotune"] = {"choice": best, "candidates": len(tried),
+                           "seconds": round(time.perf_counter() - t0, 2),
+                           "fastest_ms": min(t["ms_per_launch"] for t in tried),
+                           "slowest_ms": max(t["ms_per_launch"] for t in tried),
+                           "default_32x32_512_ms": next(
+                               (t["ms_per_launch"] for t in tried
+                                if t["tile"] == [32, 32] and t["tile_split"] == 512), None),
+                           "one_image": len({t["checksum"] for t in tried}) == 1}
+        g = ctx.batch_graph(prepared)
+        try:
+            warm_until_stable(ctx, g.launch, min_passes=2)
+            n, dt = timed_passes(ctx, g.launch, min_s=0.3, min_passes=8)
+            fps_b = n * len(frames) / dt
+            dig_b = frames_digest(ctx, outs)
+        finally:
+            g.destroy()
+        singles = [ctx.prepare_batch([s], [h], [p]) for s, h, p in zip(frames, handles, ptrs)]
+
+        def plain():
+            for pb in singles:
+                ctx.submit_batch(pb)
+
+        for o in outs:
+            o[0].fill(0)
+            o[1].fill(0)
+        warm_until_stable(ctx, plain, min_passes=2)
+        n, dt = timed_passes(ctx, plain, min_s=0.3, min_passes=4)
+        fps_p = n * len(frames) / dt
+        dig_p = frames_digest(ctx, outs)
+        ctx.set_config(pipelines=1)
+        warm_until_stable(ctx, plain, min_passes=2)
+        n, dt = timed_passes(ctx, plain, min_s=0.3, min_passes=4)
+        fps_p1 = n * len(frames) / dt
+        best_plain = max(fps_p, fps_p1)
+        res.update({
+            "frames_per_s": {"batched_graph": round(fps_b, 1), "plain": round(fps_p, 1),
+                             "plain_one_pipeline": round(fps_p1, 1)},
+            "mtris_s": {"batched_graph": round(tris / len(frames) * fps_b / 1e6, 1),
+                        "plain": round(tris / len(frames) * fps_p / 1e6, 1)},
+            "batched_over_best_plain": round(fps_b / best_plain, 3),
+            "bit_exact_batched_vs_plain": dig_b == dig_p,
+            "digest": f"{dig_b:016x}"})
+    finally:
+        ctx.set_config(tile_w=saved.tile_w, tile_h=saved.tile_h, tile_split=saved.tile_split,
+                       pipelines=saved.pipelines)
+        for o in outs:
+            o[0].free()
+            o[1].free()
+        for h in handles:
+            for m in h:
+                ctx.destroy_mesh(m)
+    return res
 
 
 def other_configs(ctx, scenes):
-    """Throughput of configs 3-5 on the same context: config 3 (64 instances,
-    ~780k triangles, 1920x1080) drawn one frame per call; config 4 (16 frames of
-    1k-500k triangles at four resolutions) as one batch; config 5 (8 requests of
-    1-8 objects over 32 resident meshes, ~8M triangles) as one batch."""
-    import numpy as np
+    """Throughput of configs 3 and 5 on the same context: config 3 (64
+    instances, ~1M triangles, 1920x1080) drawn one frame per call; config 5 (8
+    requests of 1-8 objects over 32 resident meshes, ~8M triangles) as one
+    batch. Each leg warms until its buffers stop growing, then times passes for
+    at least 0.25 s."""
     out = {}
 
-    def run(name, frames, handles, reps, batch):
+    def run(name, frames, handles, batch):
         bufs = [ctx.frame_buffers(s.frame.width, s.frame.height) for s in frames]
         ptrs = [(b[0].ptr, b[1].ptr) for b in bufs]
         tris = sum(s.num_faces for s in frames)
@@ -111,23 +265,18 @@ def other_configs(ctx, scenes):
             prepared = [ctx.prepare_batch(frames, handles, ptrs)]
         else:
             prepared = [ctx.prepare_batch([s], [h], [p]) for s, h, p in zip(frames, handles, ptrs)]
-        for _ in range(2):  # warm-up (grows tile-list capacity if needed)
+
+        def one_pass():
             for pb in prepared:
                 ctx.submit_batch(pb)
-            try:
-                ctx.synchronize()
-            except Exception:
-                pass
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            for pb in prepared:
-                ctx.submit_batch(pb)
-        ctx.synchronize()
-        dt = time.perf_counter() - t0
-        out[name] = {"mtris_s": round(tris * reps / dt / 1e6, 1),
-                     "frames_per_s": round(len(frames) * reps / dt, 1),
+
+        warm = warm_until_stable(ctx, one_pass)
+        n, dt = timed_passes(ctx, one_pass)
+        out[name] = {"mtris_s": round(tris * n / dt / 1e6, 1),
+                     "frames_per_s": round(len(frames) * n / dt, 1),
                      "triangles_per_frame_avg": int(tris / len(frames)),
-                     "form": "batched" if batch else "plain"}
+                     "form": "batched" if batch else "plain",
+                     "warmup_passes": warm, "timed_passes": n, "timed_s": round(dt, 3)}
         for b in bufs:
             b[0].free()
             b[1].free()
@@ -135,14 +284,11 @@ def other_configs(ctx, scenes):
     c3 = scenes.config3()
     th = [ctx.upload_texture(t) for t in c3.textures]
     mh = [ctx.upload_mesh(m, th) for m in c3.meshes]
-    run("config3_1080p_64_instances", [c3], [mh], 100, False)
-    c4 = scenes.config4_frames(16)
-    h4 = [ctx.upload_scene(s)[0] for s in c4]
-    run("config4_16_mixed_frames", c4, h4, 40, True)
+    run("config3_1080p_64_instances", [c3], [mh], False)
     c5 = scenes.config5(64)
     th = [ctx.upload_texture(t) for t in c5.textures]
     mh = [ctx.upload_mesh(m, th) for m in c5.meshes]
-    run("config5_8_requests_8M_resident", c5.requests[:8], [mh] * 8, 20, True)
+    run("config5_8_requests_8M_resident", c5.requests[:8], [mh] * 8, True)
     out["config5_serving"] = serving(ctx, c5.requests, mh)
     return out
 
@@ -184,6 +330,22 @@ def serving(ctx, requests, mh):
             t = time.perf_counter()
             lat, bs = replay(srv, reqs, poisson_arrivals(n, qps, seed=qps))
             res[f"qps_{qps}"] = summary(lat, time.perf_counter() - t, n, bs)
+    # The 10,000 requests/s stream exceeds capacity: with a bounded queue (64
+    # waiting) and a 20 ms admission deadline the latency of what is served
+    # stays bounded; refused and shed requests are counted beside it.
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2, max_queue=64,
+                deadline_ms=20) as srv:
+        replay(srv, reqs, np.zeros(64))
+        t = time.perf_counter()
+        lat, bs = replay(srv, reqs, poisson_arrivals(2000, 10000, seed=10000))
+        wall = time.perf_counter() - t
+        code = np.round(lat * 1e6).astype(np.int64)
+        ok = lat >= 0
+        d = summary(lat[ok], wall, int(ok.sum()), bs[ok])
+        d.update({"submitted": 2000, "refused_queue_full": int((code == -1).sum()),
+                  "shed_past_deadline": int((code == -2).sum()), "max_queue": 64,
+                  "deadline_ms": 20})
+        res["qps_10000_bounded"] = d
     # A window's present needs colour only: the same burst without depth readback.
     with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2,
                 colour_only=True) as srv:
@@ -208,12 +370,7 @@ def profile_config5(steps, warmup, tile_split, pipelines):
     mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
     bufs = [ctx.frame_buffers(s.frame.width, s.frame.height) for s in rs.requests]
     pb = ctx.prepare_batch(rs.requests, [mh] * 8, [(b[0].ptr, b[1].ptr) for b in bufs])
-    for _ in range(max(2, warmup)):
-        ctx.submit_batch(pb)
-        try:
-            ctx.synchronize()
-        except Exception:
-            pass  # the first launch grows the tile-list capacity
+    warm_until_stable(ctx, lambda: ctx.submit_batch(pb), min_passes=max(2, warmup))
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.submit_batch(pb)
@@ -237,6 +394,8 @@ def main():
                                    "(sharding frames over several GPUs is a design note, "
                                    "not built)"}))
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.batch < 1:
+        raise SystemExit("--steps >= 1, --warmup >= 0 and --batch >= 1")
 
     import numpy as np
     from gorender_amd import scenes
@@ -256,128 +415,124 @@ def main():
 
     def frame_scene(k):
         w = scenes.world_matrix((1, 1, 1), (0, 0.01 * k, 0), (0, 0, 0))
-        s = scenes.SceneData(fr, scene.meshes, scene.textures,
-                             [scenes.ObjectData(0, scenes.mvp_matrix(P, Vw, w), w)],
-                             name=f"c2-{k}")
-        return s
+        return scenes.SceneData(fr, scene.meshes, scene.textures,
+                                [scenes.ObjectData(0, scenes.mvp_matrix(P, Vw, w), w)],
+                                name=f"c2-{k}")
 
     ctx = Context(0)
     mh, _ = ctx.upload_scene(scene)
-
-    n_variants = 64
-    variants = [frame_scene(k) for k in range(n_variants)]
     B = args.batch
+    n_variants = B  # the B distinct frames every launch draws
+    variants = [frame_scene(k) for k in range(n_variants)]
     outs = [ctx.frame_buffers(W, H) for _ in range(B)]
     out_ptrs = [(o[0].ptr, o[1].ptr) for o in outs]
-
     # ctypes arguments are built before timing: the timed loops only submit.
-    batches = [ctx.prepare_batch([variants[(j * B + i) % n_variants] for i in range(B)],
-                                 [mh] * B, out_ptrs) for j in range(n_variants)]
-    singles = [ctx.prepare_batch([variants[k]], [mh], [out_ptrs[k % B]])
-               for k in range(n_variants)]
+    # Two copies of the batch, for the two graphs that alternate on the pipes.
+    batches = [ctx.prepare_batch(variants, [mh] * B, out_ptrs) for _ in range(2)]
+    singles = [ctx.prepare_batch([variants[k]], [mh], [out_ptrs[k]]) for k in range(B)]
+    tw, th = [int(v) for v in args.tile.split("x")]
+    ctx.set_config(tile_w=tw, tile_h=th, tile_split=args.tile_split, pipelines=args.pipelines)
 
-    def run_batched(k0, nframes):
-        assert nframes % B == 0
-        for j in range(k0 // B, (k0 + nframes) // B):
-            ctx.submit_batch(batches[j % n_variants])
+    step_i = [0]
 
-    def run_plain(k0, nframes):
-        for k in range(k0, k0 + nframes):
-            ctx.submit_batch(singles[k % n_variants])
+    def stream_step():
+        ctx.submit_batch(batches[step_i[0] & 1])
+        step_i[0] += 1
 
-    def set_tile(tw, th):
-        ctx.set_config(tile_w=tw, tile_h=th, tile_split=args.tile_split,
-                       pipelines=args.pipelines)
+    def plain_step():  # B frames, one launch each
+        for k in range(B):
+            ctx.submit_batch(singles[k])
+        step_i[0] += 1
 
-    tile_sweep = {}
-    if args.tile == "auto":
-        # Short timed runs of the batched form per candidate; the fastest is kept.
-        for tw, th in [(16, 16), (32, 16), (16, 32), (32, 32), (64, 16)]:
-            set_tile(tw, th)
-            run_batched(0, 4 * B)
-            ctx.synchronize()
-            t0 = time.perf_counter()
-            run_batched(0, 16 * B)
-            ctx.synchronize()
-            tile_sweep[f"{tw}x{th}"] = round(16 * B / (time.perf_counter() - t0), 1)
-        best = max(tile_sweep, key=tile_sweep.get)
-        set_tile(*[int(v) for v in best.split("x")])
-    else:
-        set_tile(*[int(v) for v in args.tile.split("x")])
-
-    do_b = args.mode in ("both", "batched")
-    do_p = args.mode in ("both", "plain")
-    steps = (max(args.steps, B) + B - 1) // B * B
-    warm = (max(args.warmup, B) + B - 1) // B * B
-    # warm-up (also grows tile-list capacity if needed); the batched form first,
-    # so that the context's buffers are allocated once at their largest size
-    def warm_up():
-        if do_b:
-            run_batched(0, warm)
-        if do_p:
-            run_plain(0, max(1, warm // 4))
-        if do_b:
-            run_batched(0, warm)
-        ctx.synchronize()
-
-    from gorender_amd import _lib
-    try:
-        warm_up()
-    except _lib.GorenderError as e:
-        if e.status != _lib.GR_ERR_CAPACITY:
-            raise
-        warm_up()  # the library has grown the tile-list capacity: draw again
-    stats = ctx.last_stats()[0]
-
-    def timed(fn, nframes):
+    def run_form(step, warmup, steps):
+        """Warm-up steps (capacity settles: a pass that overflowed is drawn
+        again), then exactly `steps` steps between two synchronisations."""
+        for _ in range(warmup):
+            capacity_safe(ctx, step)
         ctx.synchronize()
         t0 = time.perf_counter()
-        fn(0, nframes)
+        for _ in range(steps):
+            step()
         ctx.synchronize()
         return time.perf_counter() - t0
 
-    plain_steps = max(B, steps // 2)
-    wall_b = timed(run_batched, steps) if do_b else float("nan")
-    wall_p = timed(run_plain, plain_steps) if do_p else float("nan")
+    K, Wm = args.steps, args.warmup
+    modes = {"all": ("stream", "graph", "plain"), "stream": ("stream",), "graph": ("graph",),
+             "plain": ("plain",)}[args.mode]
+    # The stream form first: the context's buffers are allocated once at their
+    # largest size before the single-frame form runs.
+    capacity_safe(ctx, stream_step)
+    capacity_safe(ctx, stream_step)
+    wall = {}
+    if "stream" in modes:
+        wall["stream"] = run_form(stream_step, Wm, K)
+    graphs = []
+    if "graph" in modes:
+        graphs = [ctx.batch_graph(b) for b in batches]
 
-    # per-kernel durations (HIP events around every launch, separate pass with
-    # one pipeline, so that no two kernels overlap inside a measured duration)
+        def graph_step():
+            graphs[step_i[0] & 1].launch()
+            step_i[0] += 1
+
+        wall["graph"] = run_form(graph_step, Wm, K)
+    stats = ctx.last_stats()[0]
+    if "plain" in modes:
+        wall["plain"] = run_form(plain_step, min(Wm, 4), K)
+        ctx.set_config(pipelines=1)
+        wall["plain_one_pipeline"] = run_form(plain_step, min(Wm, 4), K)
+        ctx.set_config(pipelines=args.pipelines)
+    fps = {k: K * B / v for k, v in wall.items()}
+
+    # Per-kernel durations: HIP events around every launch, one pipeline, so
+    # that no two kernels overlap inside a measured duration.
     kt_batched, kt_plain = {}, {}
     ctx.set_config(pipelines=1)
-    # plain form strictly one frame after another (one pipeline: no two
-    # frames' kernels overlap)
-    wall_p1 = float("nan")
-    if do_p:
-        run_plain(0, 64)
-        wall_p1 = timed(run_plain, plain_steps)
-    if do_b:
+    if "stream" in modes or "graph" in modes:
         ctx.reset_kernel_times()
         ctx.set_kernel_timing(True)
-        run_batched(0, 4 * B)
+        for _ in range(4):
+            stream_step()
         ctx.synchronize()
         kt_batched = ctx.kernel_times()
         ctx.set_kernel_timing(False)
-    if do_p:
+    if "plain" in modes:
         ctx.reset_kernel_times()
         ctx.set_kernel_timing(True)
-        run_plain(0, 64)
+        plain_step()
         ctx.synchronize()
         kt_plain = ctx.kernel_times()
         ctx.set_kernel_timing(False)
     ctx.set_config(pipelines=args.pipelines)
 
+    # The device's copy bandwidth: k_copy16 over calib_copy_mib MiB, timed by
+    # HIP events (read + write bytes per second).
+    copy_gbs = None
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib << 20
         src, dst = ctx.alloc(nb), ctx.alloc(nb)
         src.fill(0x3C)
+        dst.copy_from(src)
+        ctx.synchronize()
+        ctx.reset_kernel_times()
+        ctx.set_kernel_timing(True)
         for _ in range(4):
             dst.copy_from(src)
         ctx.synchronize()
+        ms, n = ctx.kernel_times().get("k_copy", (0.0, 0))
+        ctx.set_kernel_timing(False)
+        ctx.reset_kernel_times()
+        if n:
+            copy_gbs = 2 * nb / (ms / n / 1e3) / 1e9
         src.free()
         dst.free()
 
+    c4 = None
     extra = None
     if not args.no_extra and not args.profile:
+        for g in graphs:
+            g.destroy()
+        graphs = []
+        c4 = config4_leg(ctx, scenes)
         extra = other_configs(ctx, scenes)
 
     lat_dev, lat_host = [], []
@@ -393,44 +548,35 @@ def main():
             ctx.draw(variants[k % n_variants], mh)
             lat_host.append((time.perf_counter() - t0) * 1e3)
 
-    # correctness guard on the measured workload: a frame of the timed loop
-    # against the oracle is done by tests; here record the counters only.
-    frames_s_b = steps / wall_b
-    frames_s_p = plain_steps / wall_p
-    frames_s_p1 = plain_steps / wall_p1
-    mtris_b = faces * frames_s_b / 1e6
-    mtris_p = faces * frames_s_p / 1e6
-    # the reported value is the batched form (the plain form when run alone)
-    main_fps = frames_s_b if do_b else frames_s_p
-    main_steps, main_wall = (steps, wall_b) if do_b else (plain_steps, wall_p)
+    batched = [k for k in ("graph", "stream") if k in fps]
+    main_form = max(batched, key=lambda k: fps[k]) if batched else "plain"
+    main_fps = fps[main_form]
     shaded = stats["pixels_shaded"]
     covered = stats["fragments_covered"]
     B_frame = scene.compulsory_bytes()
 
-    # roofline of the dominant kernel in the batched form
+    # Roofline of the dominant kernel in the batched form.
     kt_dom = kt_batched or kt_plain
     frames_per_launch = B if kt_batched else 1
-    dom = max(kt_dom.items(), key=lambda kv: kv[1][0])
-    dom_name, (dom_ms, dom_launches) = dom[0], dom[1]
+    dom_name, (dom_ms, dom_launches) = max(kt_dom.items(), key=lambda kv: kv[1][0])
     avg_launch_s = dom_ms / dom_launches / 1e3
     alg_bytes = {
         "k_raster": W * H * 8 + 16 * 16 * 4,
         "k_face": faces * 12 + scene.meshes[0].num_vertices * (12 + 12 + 8),
-        "k_vertex": scene.meshes[0].num_vertices * 24,
-        "k_bin_fill": 0,
-        "k_bin_scan": 0,
     }
     dom_bytes = alg_bytes.get(dom_name, 0) * frames_per_launch
     achieved = dom_bytes / avg_launch_s / 1e9 if dom_bytes else 0.0
-
-    n_valu = measured_valu(dom_name, ctx.config(), frames_per_launch)
+    sq = sq_counters(dom_name, ctx.config(), frames_per_launch)
     valu_issue = None
-    if n_valu:
-        ach = n_valu / avg_launch_s / 1e9
-        valu_issue = {"kernel": dom_name, "instr_per_launch": n_valu,
+    if sq.get("SQ_INSTS_VALU"):
+        ach = sq["SQ_INSTS_VALU"] / avg_launch_s / 1e9
+        valu_issue = {"kernel": dom_name, "instr_per_launch": int(sq["SQ_INSTS_VALU"]),
                       "achieved_ginstr_s": round(ach, 1), "peak_ginstr_s": VALU_PEAK_GINSTR_S,
                       "frac": round(ach / VALU_PEAK_GINSTR_S, 4),
                       "source": "SQ_INSTS_VALU, profiles/sq_counters_config2.txt"}
+    lds_conflict = (round(sq["SQ_LDS_BANK_CONFLICT"] / sq["SQ_ACTIVE_INST_LDS"], 4)
+                    if sq.get("SQ_ACTIVE_INST_LDS") else None)
+    occupancy = (round(sq["MEAN_WAVES_PER_SIMD"], 3) if sq.get("MEAN_WAVES_PER_SIMD") else None)
 
     cpu, cpu_forms = None, None
     if not args.no_cpu and not args.profile:
@@ -467,9 +613,9 @@ def main():
         "value": round(faces * main_fps / 1e6, 3),
         "unit": "Mtris/s",
         "n_gpus": 1,
-        "steps": main_steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(main_wall / main_steps * 1e3, 5),
+        "steps": K,
+        "warmup": Wm,
+        "ms_per_step": round(wall[main_form] / K * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -477,17 +623,18 @@ def main():
         "data": "synthetic (seeded SplitMix64 displaced sphere + 16x16 texture)",
         "config": {"workload": "headline frame: 250x400 lat-long displaced sphere, 200,000 "
                                "triangles, 16x16 RGBA8 texture, 1280x720, camera r=1.2",
+                   "step": f"one batched launch of {B} distinct frames",
+                   "form": main_form,
                    "frames_per_batch": B, "triangles_per_frame": faces,
                    "tile": [ctx.config().tile_w, ctx.config().tile_h],
                    "tile_split": ctx.config().tile_split,
                    "pipelines": ctx.config().pipelines},
         "shaded_mpix_s": round(shaded * main_fps / 1e6, 3),
         "covered_mfrag_s": round(covered * main_fps / 1e6, 3),
-        "frames_per_s": {"batched": round(frames_s_b, 2), "plain": round(frames_s_p, 2),
-                         "plain_one_pipeline": round(frames_s_p1, 2),
-                         "batched_over_plain": round(frames_s_b / frames_s_p, 3),
-                         "batched_over_plain_one_pipeline": round(frames_s_b / frames_s_p1, 3)},
-        "mtris_s": {"batched": round(mtris_b, 3), "plain": round(mtris_p, 3)},
+        "frames_per_s": {k: round(v, 2) for k, v in fps.items()},
+        "batched_over_plain": (round(main_fps / max(fps["plain"], fps["plain_one_pipeline"]), 3)
+                               if "plain" in fps else None),
+        "mtris_s": {k: round(faces * v / 1e6, 3) for k, v in fps.items()},
         "latency_ms": ({"device_p50": round(pct(lat_dev, 50), 4),
                         "device_p99": round(pct(lat_dev, 99), 4),
                         "host_p50": round(pct(lat_host, 50), 4),
@@ -505,16 +652,21 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(dom_name, ctx.config(), frames_per_launch),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "algorithmic_bytes_per_launch": dom_bytes},
+                     "algorithmic_bytes_per_launch": dom_bytes,
+                     "measured_copy_GBs": round(copy_gbs, 1) if copy_gbs else None,
+                     "lds_bank_conflicts_per_lds_instr": lds_conflict,
+                     "mean_waves_per_simd": occupancy},
         # The same kernel against its compute ceiling: VALU instructions per
         # launch (counter profile) over the live launch time.
         "valu_issue": valu_issue,
         "cpu_baseline": cpu,
         "cpu_baseline_forms": cpu_forms,
-        "tile_sweep_fps": tile_sweep or None,
+        "config4_256_mixed_frames": c4,
         "other_configs": extra,
     }
     print(json.dumps(out))
+    for g in graphs:
+        g.destroy()
     ctx.close()
 
 

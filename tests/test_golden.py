@@ -1,6 +1,6 @@
 """The oracle reproduces its committed digests (tests/golden/oracle_digests.json)
-for every golden scene: the small cases, configs 1-3, 16 frames of config 4
-and 8 requests of config 5. Regenerate with tests/golden/make_golden.py only
+for every golden scene: the small cases, configs 1-3, all 256 frames of
+config 4 and all 64 requests of config 5. Regenerate with tests/golden/make_golden.py only
 when a deliberate change to the oracle or a generator is made."""
 import json
 import os
