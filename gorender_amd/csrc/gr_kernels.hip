@@ -406,8 +406,10 @@ __device__ int clip_one_plane(int p, const CV& c0, const CV& c1, const CV& c2, f
 
 // Per-lane counters of k_face packed in one word (one register for the whole
 // kernel): TPF references in bits 0-15, binned triangles 16-23, front faces
-// 24-27, clip overflows 28-31. A lane owns one face in pass 1 and emits at most
-// a few triangles of at most 16 references each, far below these widths.
+// 24-27, clip overflows 28-31. A lane owns kFacesPerLane faces in pass 1 (at
+// most 4 front faces) and, over at most 4 rounds of pass 2, at most 4 clip
+// overflows and ~40 triangles of at most 16 references each: below these
+// widths, and a wave's sums of the 16-bit fields stay below 2^16.
 struct FaceStats {
     uint32_t packed;
 };
@@ -877,29 +879,39 @@ __device__ __forceinline__ float normal_intensity(const DevObject& o, const DevF
     return lambert(xform(o.world, q[0], q[1], q[2], w), fr.light, fr.ambient, fr.diffuse);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_face(const DevObject* __restrict__ objs, int nobj,
+// LDS writes of one wave made visible to the wave's other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One face block (kFacesPerWg faces of one object) per single-wave workgroup,
+// kFacesPerLane faces per lane. A block's few survivors keep one wave busy
+// through the long per-survivor chain of pass 2 (dependent loads, lighting,
+// clipping, set-up), so the workgroup is that one wave: no sibling waves wait
+// at barriers holding registers, and every resident wave works.
+constexpr int kFacesPerLane = kFacesPerWg / 64;
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_face(const DevObject* __restrict__ objs, int nobj,
                                               const DevFrame* __restrict__ frames,
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
                                               unsigned long long* __restrict__ stats, int twl,
                                               int thl, const int* __restrict__ xcd_face,
                                               int nframes) {
-    __shared__ int lds_scan[4];
-    // Per-wave counters: TPF references | binned triangles << 16, front faces
-    // | clip overflows << 16 (each field stays below 2^16 for 64 faces).
-    __shared__ uint32_t lds_cnt[4][2];
-    __shared__ int lds_nsurv[4];                     // survivors per wave
-    __shared__ uint32_t lds_sface[kFacesPerWg];      // survivor: face offset | plane << 8
-    __shared__ float lds_spos[kFacesPerWg][12];      // survivor: clip-space corners
-    __shared__ CV wpoly[4][2][kMaxClip];
-    __shared__ CV lds_stage[4][kStage][kMaxClip];  // clipped polygons of general faces
-    __shared__ int lds_stage_k[4][kStage], lds_stage_tex[4][kStage], lds_stage_slot[4][kStage];
+    static_assert(kFacesPerWg == 64 * kFacesPerLane, "whole faces per lane");
+    // Survivors in face order: face offset | first clip plane << 8.
+    __shared__ uint32_t lds_sface[kFacesPerWg];
+    __shared__ CV wpoly[2][kMaxClip];
+    __shared__ CV lds_stage[kStage][kMaxClip];  // clipped polygons of general faces
+    __shared__ int lds_stage_k[kStage], lds_stage_tex[kStage], lds_stage_slot[kStage];
     __shared__ uint32_t lds_tile_tag[kTileTable], lds_tile_cnt[kTileTable];
 
     int bf, blocal;
     if (!xcd_unit(xcd_face, nframes, (int)blockIdx.x, bf, blocal)) return;
-    // The barrier of the survivor compaction orders this before any count.
-    for (int i = threadIdx.x; i < kTileTable; i += 256) {
+    const int lane = (int)threadIdx.x;
+    for (int i = lane; i < kTileTable; i += 64) {
         lds_tile_tag[i] = kNoTile;
         lds_tile_cnt[i] = 0u;
     }
@@ -914,223 +926,209 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     const DevMesh& m = o.m;
     const DevFrame& fr = frames[o.frame];
     const int wg_in_obj = b - o.face_block;
-    const int face = wg_in_obj * kFacesPerWg + threadIdx.x;
     const uint32_t flags = fr.flags;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int vis = o.vis;  // BoxVisibility, per object on the host
 
     FaceStats st = {0u};
 
-    // Pass 1, one face per thread: clip-space corners, backface test and plane
-    // classification. The survivors (front faces the clipper does not reject
-    // outright) are compacted in face order into LDS with their corners, so the
-    // lighting, clipping and set-up below run on dense waves.
-    bool survivor = false;
-    int p = 6;
-    float4 v0, v1, v2;
-    if (face < m.F && vis != kBoxOutside) {
-        const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
-        v0 = corner(o, i0);
-        v1 = corner(o, i1);
-        v2 = corner(o, i2);
-        if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
-            st.packed += 1u << 24;
-            uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
-            if ((flags & kFrustumClipping) && vis != kBoxInside) {
-                m0 = inside_mask(CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
-                m1 = inside_mask(CV{v1.x, v1.y, v1.z, v1.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
-                m2 = inside_mask(CV{v2.x, v2.y, v2.z, v2.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+    // Pass 1, faces lane, lane + 64, ... of the block: clip-space corners,
+    // backface test and plane classification. The survivors (front faces the
+    // clipper does not reject outright) are compacted in face order into LDS,
+    // so the lighting, clipping and set-up below run on dense lanes.
+    int nsurv = 0;  // wave-uniform
+    for (int k = 0; k < kFacesPerLane; k++) {
+        const int fo = k * 64 + lane;  // face offset in the block
+        const int face = wg_in_obj * kFacesPerWg + fo;
+        bool survivor = false;
+        int p = 6;
+        if (face < m.F && vis != kBoxOutside) {
+            const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
+            const float4 v0 = corner(o, i0), v1 = corner(o, i1), v2 = corner(o, i2);
+            if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
+                st.packed += 1u << 24;
+                uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
+                if ((flags & kFrustumClipping) && vis != kBoxInside) {
+                    m0 = inside_mask(CV{v0.x, v0.y, v0.z, v0.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+                    m1 = inside_mask(CV{v1.x, v1.y, v1.z, v1.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+                    m2 = inside_mask(CV{v2.x, v2.y, v2.z, v2.w, 0.f, 0.f, 0.f}, fr.znear, fr.zfar);
+                }
+                const uint32_t all_in = m0 & m1 & m2;
+                // First plane not passed by every vertex (6 if none). Earlier
+                // planes leave the triangle untouched; when this one rejects
+                // every vertex the pass yields an empty polygon.
+                p = __builtin_ctz(~all_in | 0x40u);
+                survivor = !(p < 6 && !(((m0 | m1 | m2) >> p) & 1u));
             }
-            const uint32_t all_in = m0 & m1 & m2;
-            // First plane not passed by every vertex (6 if none). Earlier planes
-            // leave the triangle untouched; when this one rejects every vertex
-            // the pass yields an empty polygon.
-            p = __builtin_ctz(~all_in | 0x40u);
-            survivor = !(p < 6 && !(((m0 | m1 | m2) >> p) & 1u));
         }
-    }
-    {
         const uint64_t sm = __ballot(survivor);
-        if (lane == 0) lds_nsurv[wave] = __popcll(sm);
-        __syncthreads();
-        int base = 0;
-        for (int w = 0; w < wave; w++) base += lds_nsurv[w];
-        if (survivor) {
-            const int si = base + __popcll(sm & lanemask_lt(lane));
-            lds_sface[si] = (uint32_t)threadIdx.x | ((uint32_t)p << 8);
-            float4* dst = reinterpret_cast<float4*>(lds_spos[si]);
-            dst[0] = v0;
-            dst[1] = v1;
-            dst[2] = v2;
-        }
-        __syncthreads();
+        if (survivor)
+            lds_sface[nsurv + __popcll(sm & lanemask_lt(lane))] = (uint32_t)fo | ((uint32_t)p << 8);
+        nsurv += __popcll(sm);
     }
-    const int nsurv = lds_nsurv[0] + lds_nsurv[1] + lds_nsurv[2] + lds_nsurv[3];
+    wave_sync();
 
-    // Pass 2, survivor s on thread s: lighting, texture coordinates, clipping.
-    // 0: nothing, 1: emit the triangle as is, 2: full clipper in the wave,
-    // 3: one-plane polygon. q[0..nq-1] holds the polygon to emit (the
+    // Pass 2, survivor r0 + lane, 64 a round: lighting, texture coordinates,
+    // clipping. 0: nothing, 1: emit the triangle as is, 2: full clipper in the
+    // wave, 3: one-plane polygon. q[0..nq-1] holds the polygon to emit (the
     // triangle itself for modes 1 and 2), so only one set of corners stays
     // live across the emits.
-    int mode = 0;
-    int n = 0;
-    CV q[4];
-    int nq = 0;
-    int tex = -1;
-    if ((int)threadIdx.x < nsurv) {
-        CV c0, c1, c2;
-        const uint32_t e = lds_sface[threadIdx.x];
-        const int sf = wg_in_obj * kFacesPerWg + (int)(e & 0xFFu);
-        const int sp = (int)(e >> 8);
-        const float4* src = reinterpret_cast<const float4*>(lds_spos[threadIdx.x]);
-        const float4 w0 = src[0], w1 = src[1], w2 = src[2];
-        c0 = CV{w0.x, w0.y, w0.z, w0.w, 0.f, 0.f, 0.f};
-        c1 = CV{w1.x, w1.y, w1.z, w1.w, 0.f, 0.f, 0.f};
-        c2 = CV{w2.x, w2.y, w2.z, w2.w, 0.f, 0.f, 0.f};
-        const int i0 = m.vi[3 * sf + 0], i1 = m.vi[3 * sf + 1], i2 = m.vi[3 * sf + 2];
-        float in0, in1, in2;
-        if (flags & kLighting) {
-            if (o.gouraud) {
-                int n0 = i0, n1 = i1, n2 = i2;
-                if (m.ni) {
-                    n0 = m.ni[3 * sf + 0];
-                    n1 = m.ni[3 * sf + 1];
-                    n2 = m.ni[3 * sf + 2];
+    const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
+    uint32_t run = 0;  // slots of the earlier rounds (wave-uniform)
+    for (int r0 = 0; r0 < nsurv; r0 += 64) {
+        int mode = 0;
+        int n = 0;
+        CV q[4];
+        int nq = 0;
+        int tex = -1;
+        if (r0 + lane < nsurv) {
+            CV c0, c1, c2;
+            const uint32_t e = lds_sface[r0 + lane];
+            const int sf = wg_in_obj * kFacesPerWg + (int)(e & 0xFFu);
+            const int sp = (int)(e >> 8);
+            const int i0 = m.vi[3 * sf + 0], i1 = m.vi[3 * sf + 1], i2 = m.vi[3 * sf + 2];
+            // The corners again (the same arithmetic as pass 1).
+            const float4 w0 = corner(o, i0), w1 = corner(o, i1), w2 = corner(o, i2);
+            c0 = CV{w0.x, w0.y, w0.z, w0.w, 0.f, 0.f, 0.f};
+            c1 = CV{w1.x, w1.y, w1.z, w1.w, 0.f, 0.f, 0.f};
+            c2 = CV{w2.x, w2.y, w2.z, w2.w, 0.f, 0.f, 0.f};
+            float in0, in1, in2;
+            if (flags & kLighting) {
+                if (o.gouraud) {
+                    int n0 = i0, n1 = i1, n2 = i2;
+                    if (m.ni) {
+                        n0 = m.ni[3 * sf + 0];
+                        n1 = m.ni[3 * sf + 1];
+                        n2 = m.ni[3 * sf + 2];
+                    }
+                    in0 = normal_intensity(o, fr, n0);
+                    in1 = normal_intensity(o, fr, n1);
+                    in2 = normal_intensity(o, fr, n2);
+                } else {
+                    const float* f = m.fn + 4 * (size_t)sf;
+                    float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
+                    in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
                 }
-                in0 = normal_intensity(o, fr, n0);
-                in1 = normal_intensity(o, fr, n1);
-                in2 = normal_intensity(o, fr, n2);
             } else {
-                const float* f = m.fn + 4 * (size_t)sf;
-                float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
-                in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
+                in0 = in1 = in2 = fr.ambient;
             }
-        } else {
-            in0 = in1 = in2 = fr.ambient;
-        }
-        c0.i = in0;
-        c1.i = in1;
-        c2.i = in2;
-        if (m.uv_vert) {
-            c0.u = m.uv_vert[2 * i0];
-            c0.v = m.uv_vert[2 * i0 + 1];
-            c1.u = m.uv_vert[2 * i1];
-            c1.v = m.uv_vert[2 * i1 + 1];
-            c2.u = m.uv_vert[2 * i2];
-            c2.v = m.uv_vert[2 * i2 + 1];
-        } else if (m.uv_face) {
-            const float* uv = m.uv_face + 6 * (size_t)sf;
-            c0.u = uv[0];
-            c0.v = uv[1];
-            c1.u = uv[2];
-            c1.v = uv[3];
-            c2.u = uv[4];
-            c2.v = uv[5];
-        }
-        if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[sf] : m.mesh_tex;
-        if (sp == 6) {
-            // Every pass only rotates the triangle; after 6 passes it is
-            // unchanged (clipping.go:199-232); no clipping: as is.
-            mode = 1;
-            n = 1;
-            nq = 3;
-            q[0] = c0;
-            q[1] = c1;
-            q[2] = c2;
-        } else {
-            nq = clip_one_plane(sp, c0, c1, c2, fr.znear, fr.zfar, q);
-            if (nq > 0) {
-                mode = 3;
-                n = nq - 2;
-            } else if (nq < 0) {
-                nq = 0;
-                mode = 2;  // counted below, one face at a time per wave
+            c0.i = in0;
+            c1.i = in1;
+            c2.i = in2;
+            if (m.uv_vert) {
+                c0.u = m.uv_vert[2 * i0];
+                c0.v = m.uv_vert[2 * i0 + 1];
+                c1.u = m.uv_vert[2 * i1];
+                c1.v = m.uv_vert[2 * i1 + 1];
+                c2.u = m.uv_vert[2 * i2];
+                c2.v = m.uv_vert[2 * i2 + 1];
+            } else if (m.uv_face) {
+                const float* uv = m.uv_face + 6 * (size_t)sf;
+                c0.u = uv[0];
+                c0.v = uv[1];
+                c1.u = uv[2];
+                c1.v = uv[3];
+                c2.u = uv[4];
+                c2.v = uv[5];
+            }
+            if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[sf] : m.mesh_tex;
+            if (sp == 6) {
+                // Every pass only rotates the triangle; after 6 passes it is
+                // unchanged (clipping.go:199-232); no clipping: as is.
+                mode = 1;
+                n = 1;
+                nq = 3;
                 q[0] = c0;
                 q[1] = c1;
                 q[2] = c2;
+            } else {
+                nq = clip_one_plane(sp, c0, c1, c2, fr.znear, fr.zfar, q);
+                if (nq > 0) {
+                    mode = 3;
+                    n = nq - 2;
+                } else if (nq < 0) {
+                    nq = 0;
+                    mode = 2;  // counted below, one face at a time
+                    q[0] = c0;
+                    q[1] = c1;
+                    q[2] = c2;
+                }
             }
         }
-    }
 
-    // Faces for the full clipper reserve the largest fan (kMaxFan slots): the
-    // slots their polygon leaves unused get empty records, which are never
-    // binned, so slots stay in draw order and each face is clipped once.
-    if (mode == 2) n = kMaxFan;
-    int total;
-    const int off = fan_exclusive_scan(n, lds_scan, total);
-    const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
-    const uint32_t slot0 = wg_slot0 + (uint32_t)off;
-    // Fan of q (Polygon.Triangulate, clipping.go:44-62): (P0, P1, P2), then
-    // (P0, P2, P3) for a quad.
-    if (mode == 1 || mode == 3)
-        add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tally, twl, thl));
-    if (mode == 3 && nq == 4)
-        add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl, thl));
+        // Faces for the full clipper reserve the largest fan (kMaxFan slots):
+        // the slots their polygon leaves unused get empty records, which are
+        // never binned, so slots stay in draw order and each face is clipped
+        // once. The prefix of the fan sizes (0, 1, 2 or kMaxFan) in face order
+        // is counted from three ballots.
+        if (mode == 2) n = kMaxFan;
+        const uint64_t b1 = __ballot(n == 1), b2 = __ballot(n == 2), b7 = __ballot(n == kMaxFan);
+        const uint64_t lt = lanemask_lt(lane);
+        const uint32_t slot0 = wg_slot0 + run +
+                               (uint32_t)(__popcll(b1 & lt) + 2 * __popcll(b2 & lt) +
+                                          kMaxFan * __popcll(b7 & lt));
+        run += (uint32_t)(__popcll(b1) + 2 * __popcll(b2) + kMaxFan * __popcll(b7));
+        // Fan of q (Polygon.Triangulate, clipping.go:44-62): (P0, P1, P2),
+        // then (P0, P2, P3) for a quad.
+        if (mode == 1 || mode == 3)
+            add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tally, twl, thl));
+        if (mode == 3 && nq == 4)
+            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl, thl));
 
-    // General faces, kStage at a time per wave: the wave clips each one into
-    // its stage (clipping.go:167-236), then one lane per fan slot emits the
-    // triangle (P0, P_t+1, P_t+2) of Polygon.Triangulate (clipping.go:44-62)
-    // or an empty record.
-    CV* buf0 = wpoly[wave][0];
-    CV* buf1 = wpoly[wave][1];
-    for (uint64_t gen = __ballot(mode == 2); gen;) {
-        int nst = 0;
-        for (; gen && nst < kStage; gen &= gen - 1, nst++) {
-            const int L = __builtin_ctzll(gen);
-            const CV a = shfl_cv(q[0], L), bb = shfl_cv(q[1], L), c = shfl_cv(q[2], L);
-            const int texL = __shfl(tex, L, 64);
-            const int sL = __shfl((int)slot0, L, 64);
-            const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
-            if (k < 0 && lane == L) st.packed += 1u << 28;
-            if (lane < k) lds_stage[wave][nst][lane] = buf0[lane];
-            if (lane == 0) {
-                lds_stage_k[wave][nst] = k;
-                lds_stage_tex[wave][nst] = texL;
-                lds_stage_slot[wave][nst] = sL;
+        // General faces, kStage at a time: the wave clips each one into its
+        // stage (clipping.go:167-236), then one lane per fan slot emits the
+        // triangle (P0, P_t+1, P_t+2) of Polygon.Triangulate
+        // (clipping.go:44-62) or an empty record.
+        CV* buf0 = wpoly[0];
+        CV* buf1 = wpoly[1];
+        for (uint64_t gen = __ballot(mode == 2); gen;) {
+            int nst = 0;
+            for (; gen && nst < kStage; gen &= gen - 1, nst++) {
+                const int L = __builtin_ctzll(gen);
+                const CV a = shfl_cv(q[0], L), bb = shfl_cv(q[1], L), c = shfl_cv(q[2], L);
+                const int texL = __shfl(tex, L, 64);
+                const int sL = __shfl((int)slot0, L, 64);
+                const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
+                if (k < 0 && lane == L) st.packed += 1u << 28;
+                if (lane < k) lds_stage[nst][lane] = buf0[lane];
+                if (lane == 0) {
+                    lds_stage_k[nst] = k;
+                    lds_stage_tex[nst] = texL;
+                    lds_stage_slot[nst] = sL;
+                }
+                wave_sync();
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int r = lane / kMaxFan, t = lane - r * kMaxFan;
+            if (r < nst) {
+                const int k = lds_stage_k[r];
+                const uint32_t slot = (uint32_t)lds_stage_slot[r] + (uint32_t)t;
+                const CV* P = lds_stage[r];
+                if (t < k - 2)
+                    add_emit(st, emit_triangle(fr, P[0], P[t + 1], P[t + 2], lds_stage_tex[r],
+                                               slot, tally, twl, thl));
+                else
+                    empty_record(fr, slot);
+            }
+            wave_sync();
         }
-        const int r = lane / kMaxFan, t = lane - r * kMaxFan;
-        if (r < nst) {
-            const int k = lds_stage_k[wave][r];
-            const uint32_t slot = (uint32_t)lds_stage_slot[wave][r] + (uint32_t)t;
-            const CV* P = lds_stage[wave][r];
-            if (t < k - 2)
-                add_emit(st, emit_triangle(fr, P[0], P[t + 1], P[t + 2], lds_stage_tex[wave][r],
-                                           slot, tally, twl, thl));
-            else
-                empty_record(fr, slot);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (threadIdx.x == 0) wg_count[b] = (uint32_t)total;
+    if (lane == 0) wg_count[b] = run;
 
-    // Counters per wave: front faces and clip overflows are one bit per lane
-    // (ballot counts), TPF references and binned triangles one 32-bit sum;
-    // then one global atomic per counter per workgroup.
+    // Counters, two wave sums of 16-bit fields (TPF references | binned
+    // triangles; front faces | clip overflows), then one global atomic per
+    // counter per block.
     {
         const uint32_t c0 = wave_sum_u32((st.packed & 0xFFFFu) | (((st.packed >> 16) & 0xFFu) << 16));
-        const uint32_t nf = (uint32_t)__popcll(__ballot(((st.packed >> 24) & 0xFu) != 0u));
-        const uint32_t nc = (uint32_t)__popcll(__ballot((st.packed >> 28) != 0u));
-        if (lane == 0) {
-            lds_cnt[wave][0] = c0;
-            lds_cnt[wave][1] = nf | (nc << 16);
+        const uint32_t c1 = wave_sum_u32(((st.packed >> 24) & 0xFu) | ((st.packed >> 28) << 16));
+        if (lane < 4) {
+            const uint32_t w = lane < 2 ? c0 : c1;
+            const unsigned long long t = (w >> ((lane & 1) * 16)) & 0xFFFFu;
+            const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
+            if (t) atomicAdd(&stats[o.frame * kNumStats + idx[lane]], t);
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const int wd = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 16;
-        const unsigned long long t = ((lds_cnt[0][wd] >> sh) & 0xFFFFu) + ((lds_cnt[1][wd] >> sh) & 0xFFFFu) +
-                                     ((lds_cnt[2][wd] >> sh) & 0xFFFFu) + ((lds_cnt[3][wd] >> sh) & 0xFFFFu);
-        const int idx[4] = {kStatTpf, kStatTriangles, kStatFront, kStatClipOverflow};
-        if (t) atomicAdd(&stats[o.frame * kNumStats + idx[threadIdx.x]], t);
-    }
-    // The barrier above also ends every count: flush the table.
-    for (int i = threadIdx.x; i < kTileTable; i += 256) {
+    // Every count is in the table: flush it.
+    wave_sync();
+    for (int i = lane; i < kTileTable; i += 64) {
         const uint32_t v = lds_tile_cnt[i];
         if (v) atomicAdd(&tile_counts[lds_tile_tag[i]], v);
     }
@@ -2025,7 +2023,7 @@ void launch_face_normals(const float* pos, const int32_t* vi, int F, float* fn, 
 
 void launch_face(const LaunchParams& p, hipStream_t s) {
     if (p.face_blocks == 0) return;
-    hipLaunchKernelGGL(k_face, dim3(p.face_grid), dim3(256), 0, s, p.objs, p.nobj, p.frames,
+    hipLaunchKernelGGL(k_face, dim3(p.face_grid), dim3(64), 0, s, p.objs, p.nobj, p.frames,
                        p.tile_counts, p.wg_count, p.stats, p.tw_log2, p.th_log2, p.xcd_face,
                        p.nframes);
 }
