@@ -936,15 +936,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // clipper does not reject outright) are compacted in face order into LDS,
     // so the lighting, clipping and set-up below run on dense lanes.
     int nsurv = 0;  // wave-uniform
-    for (int k = 0; k < kFacesPerLane; k++) {
-        const int fo = k * 64 + lane;  // face offset in the block
-        const int face = wg_in_obj * kFacesPerWg + fo;
-        bool survivor = false;
-        int p = 6;
-        if (face < m.F && vis != kBoxOutside) {
-            const int i0 = m.vi[3 * face + 0], i1 = m.vi[3 * face + 1], i2 = m.vi[3 * face + 2];
-            const float4 v0 = corner(o, i0), v1 = corner(o, i1), v2 = corner(o, i2);
-            if (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2)) {
+    if (vis != kBoxOutside) {
+        // All the lane's vertex indices, then all its corners, are loaded
+        // before any is used: lanes past the mesh's last face read its last
+        // face (in bounds) and are not counted.
+        const int face0 = wg_in_obj * kFacesPerWg + lane;
+        int iv[kFacesPerLane][3];
+#pragma unroll
+        for (int k = 0; k < kFacesPerLane; k++) {
+            const int face = min(face0 + 64 * k, m.F - 1);
+            iv[k][0] = m.vi[3 * face + 0];
+            iv[k][1] = m.vi[3 * face + 1];
+            iv[k][2] = m.vi[3 * face + 2];
+        }
+        float4 cv[kFacesPerLane][3];
+#pragma unroll
+        for (int k = 0; k < kFacesPerLane; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cv[k][c] = corner(o, iv[k][c]);
+#pragma unroll
+        for (int k = 0; k < kFacesPerLane; k++) {
+            const int fo = k * 64 + lane;  // face offset in the block
+            const float4 v0 = cv[k][0], v1 = cv[k][1], v2 = cv[k][2];
+            bool survivor = false;
+            int p = 6;
+            if (face0 + 64 * k < m.F &&
+                (!(flags & kBackfaceCulling) || facing_camera(v0, v1, v2))) {
                 st.packed += 1u << 24;
                 uint32_t m0 = 0x3Fu, m1 = 0x3Fu, m2 = 0x3Fu;
                 if ((flags & kFrustumClipping) && vis != kBoxInside) {
@@ -959,11 +976,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 p = __builtin_ctz(~all_in | 0x40u);
                 survivor = !(p < 6 && !(((m0 | m1 | m2) >> p) & 1u));
             }
+            const uint64_t sm = __ballot(survivor);
+            if (survivor)
+                lds_sface[nsurv + __popcll(sm & lanemask_lt(lane))] = (uint32_t)fo | ((uint32_t)p << 8);
+            nsurv += __popcll(sm);
         }
-        const uint64_t sm = __ballot(survivor);
-        if (survivor)
-            lds_sface[nsurv + __popcll(sm & lanemask_lt(lane))] = (uint32_t)fo | ((uint32_t)p << 8);
-        nsurv += __popcll(sm);
     }
     wave_sync();
 
