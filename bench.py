@@ -321,7 +321,7 @@ def serving(ctx, requests, mh):
         lat.append((time.perf_counter() - t) * 1e3)
     res = {"serial": summary(lat, time.perf_counter() - t0, len(requests))}
     reqs = [PreparedRequest(s, mh) for s in requests]
-    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2) as srv:
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=3) as srv:
         replay(srv, reqs, np.zeros(64))  # warm-up: target pools
         t = time.perf_counter()
         lat, bs = replay(srv, reqs, np.zeros(512))
@@ -333,7 +333,7 @@ def serving(ctx, requests, mh):
     # The 10,000 requests/s stream exceeds capacity: with a bounded queue (64
     # waiting) and a 20 ms admission deadline the latency of what is served
     # stays bounded; refused and shed requests are counted beside it.
-    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2, max_queue=64,
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=3, max_queue=64,
                 deadline_ms=20) as srv:
         replay(srv, reqs, np.zeros(64))
         t = time.perf_counter()
@@ -347,13 +347,13 @@ def serving(ctx, requests, mh):
                   "deadline_ms": 20})
         res["qps_10000_bounded"] = d
     # A window's present needs colour only: the same burst without depth readback.
-    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=2,
+    with Server(ctx, target_budget=2 << 30, max_batch=16, max_inflight=3,
                 colour_only=True) as srv:
         replay(srv, reqs, np.zeros(64))
         t = time.perf_counter()
         lat, bs = replay(srv, reqs, np.zeros(512))
         res["saturated_colour_only"] = summary(lat, time.perf_counter() - t, 512, bs)
-    res["config"] = {"target_budget_bytes": 2 << 30, "max_batch": 16, "max_inflight": 2,
+    res["config"] = {"target_budget_bytes": 2 << 30, "max_batch": 16, "max_inflight": 3,
                      "distinct_requests": len(requests)}
     return res
 

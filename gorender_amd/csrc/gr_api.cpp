@@ -852,10 +852,12 @@ gr_status present_enqueue(gr_context* c, gr_context::PresentSlot& ps) {
     gr_status st = enqueue(c, &bf, 1, ps.objs.data(), (int)ps.objs.size());
     if (st) return st;
     const hipStream_t s = c->pipes[c->last_pipe].stream;
-    GR_HIP(hipMemcpyAsync(ps.host, ps.rgba.p, npx * 4, hipMemcpyDeviceToHost, s));
-    if (ps.with_depth)
-        GR_HIP(hipMemcpyAsync((char*)ps.host + npx * 4, ps.depth.p, npx * 4,
-                              hipMemcpyDeviceToHost, s));
+    // The pixels by a copy kernel writing the pinned pages (the other pipe's
+    // draw runs meanwhile, and DMA copies then reach about half the link's
+    // rate); the few words of stats and flags by DMA.
+    gr::launch_copy(ps.host, ps.rgba.p, npx * 4, s);
+    if (ps.with_depth) gr::launch_copy((char*)ps.host + npx * 4, ps.depth.p, npx * 4, s);
+    GR_HIP(hipGetLastError());
     GR_HIP(hipMemcpyAsync(ps.h_meta, c->last_stats, gr::kNumStats * 8, hipMemcpyDeviceToHost, s));
     GR_HIP(hipMemcpyAsync((char*)ps.h_meta + gr::kNumStats * 8, c->last_overflow, 4,
                           hipMemcpyDeviceToHost, s));

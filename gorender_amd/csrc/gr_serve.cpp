@@ -28,6 +28,7 @@
 #include "gorender_hip.h"
 #include "gr_error.h"
 #include "gr_internal.h"
+#include "gr_launch.h"
 
 namespace {
 
@@ -193,13 +194,15 @@ gr_status gr_server::launch(std::vector<Request*>& reqs) {
         return fail(GR_ERR_DEVICE, "server batch readback");
     }
     // Frames to pinned host memory on the copy stream, overlapping the next
-    // batch: colour, then depth unless the server reads back colour only.
-    for (Request* r : reqs) {
-        if (hipMemcpyAsync(r->target.host, r->target.dev, r->target.host_bytes,
-                           hipMemcpyDeviceToHost, copy_stream) != hipSuccess) {
-            spare.push_back(b);
-            return fail(GR_ERR_DEVICE, "server frame readback");
-        }
+    // batch: colour, then depth unless the server reads back colour only. A
+    // copy kernel writes the pinned pages over PCIe: while the next batch
+    // renders, DMA copies of the same frames ran at about half the link's
+    // rate, the kernel near its full rate.
+    for (Request* r : reqs)
+        gr::launch_copy(r->target.host, r->target.dev, r->target.host_bytes, copy_stream);
+    if (hipGetLastError() != hipSuccess) {
+        spare.push_back(b);
+        return fail(GR_ERR_DEVICE, "server frame readback");
     }
     if (hipEventRecord(b.copied, copy_stream) != hipSuccess) {
         spare.push_back(b);

@@ -45,7 +45,7 @@ class PreparedRequest:
 
 class Server:
     def __init__(self, ctx: Context, target_budget: int = 1 << 30, max_batch: int = 16,
-                 max_inflight: int = 2, colour_only: bool = False, max_queue: int = 0,
+                 max_inflight: int = 3, colour_only: bool = False, max_queue: int = 0,
                  deadline_ms: int = 0):
         self.ctx = ctx
         self.lib = ctx.lib
