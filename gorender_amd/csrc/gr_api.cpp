@@ -106,6 +106,8 @@ struct gr_context {
     Staging ring[4];
     int ring_i = 0;
     void* h_ctrl = nullptr;   // pinned: sticky control words
+    void* h_frame = nullptr;  // pinned: gr_draw's colour + depth on their way to the caller
+    size_t h_frame_cap = 0;
     int32_t last_frames = 0;
     const unsigned long long* last_stats = nullptr;  // device stats of the last draw
     // Control words of the no-allocation draw (in its fixed device array),
@@ -1266,6 +1268,7 @@ void gr_destroy(gr_context* c) {
         free_pipe_bufs(pp);
         if (pp.raster_done) (void)hipEventDestroy(pp.raster_done);
     }
+    if (c->h_frame) (void)hipHostFree(c->h_frame);
     for (auto& ps : c->present) {
         if (ps.rgba.p) (void)hipFree(ps.rgba.p);
         if (ps.depth.p) (void)hipFree(ps.depth.p);
@@ -1648,6 +1651,17 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
         size_t npx = (size_t)frame->width * frame->height;
         if ((st = ensure(c, c->out_rgba, npx * 4))) return st;
         if ((st = ensure(c, c->out_depth, npx * 4))) return st;
+        // The caller's buffers are pageable: the frame goes to pinned memory
+        // by a copy kernel (near the PCIe link's rate), then to them by a host
+        // copy, instead of a synchronous device-to-pageable copy.
+        if (c->h_frame_cap < npx * 8) {
+            if (c->h_frame) GR_HIP(hipHostFree(c->h_frame));
+            c->h_frame = nullptr;
+            c->h_frame_cap = 0;
+            GR_HIP(hipHostMalloc(&c->h_frame, npx * 8, hipHostMallocDefault));
+            c->h_frame_cap = npx * 8;
+        }
+        char* const staged = (char*)c->h_frame;
         // Synchronous: each frame is waited for, so there is nothing to overlap.
         struct OnePipe {
             gr_context* c;
@@ -1657,6 +1671,11 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
         for (int attempt = 0; attempt < 4; attempt++) {
             st = gr_draw_device(c, frame, objs, no, c->out_rgba.p, c->out_depth.p);
             if (st) return st;
+            // The copies follow the draw on its stream; a draw that overflowed
+            // is drawn again and its copies are discarded.
+            if (rgba) gr::launch_copy(staged, c->out_rgba.p, npx * 4, c->stream);
+            if (depth) gr::launch_copy(staged + npx * 4, c->out_depth.p, npx * 4, c->stream);
+            GR_HIP(hipGetLastError());
             unsigned long long needed = 0;
             st = sync_and_check(c, &needed);
             if (st == GR_ERR_CAPACITY) {
@@ -1664,8 +1683,8 @@ gr_status gr_draw(gr_context* c, const gr_frame* frame, const gr_object* objs, i
                 continue;
             }
             if (st) return st;
-            if (rgba) GR_HIP(hipMemcpy(rgba, c->out_rgba.p, npx * 4, hipMemcpyDeviceToHost));
-            if (depth) GR_HIP(hipMemcpy(depth, c->out_depth.p, npx * 4, hipMemcpyDeviceToHost));
+            if (rgba) std::memcpy(rgba, staged, npx * 4);
+            if (depth) std::memcpy(depth, staged + npx * 4, npx * 4);
             if (stats) {
                 std::vector<unsigned long long> sv;
                 if ((st = read_stats(c, sv))) return st;
