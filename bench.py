@@ -381,6 +381,38 @@ def profile_config5(steps, warmup, tile_split, pipelines):
                       "tiles_split": sum(x["tiles_split"] for x in st)}))
 
 
+def present_rates(ctx, variants, mh, W, H, n=256):
+    """Frames per second through gr_present_draw / gr_present_wait with two
+    frames in flight, as a window loop runs them (main.go:201-227): frame k+1
+    is queued before frame k is waited for; each wait leaves the frame's pixels
+    in pinned host memory (read where they lie, no further copy). Colour only
+    (what a window presents), and colour with depth."""
+    import ctypes
+    from gorender_amd import _lib
+    rp = ctypes.POINTER(ctypes.c_uint8)()
+    dp = ctypes.POINTER(ctypes.c_float)()
+    st = _lib.gr_stats()
+
+    def wait():
+        _lib.check(ctx.lib.gr_present_wait(ctx.handle, ctypes.byref(rp), ctypes.byref(dp),
+                                           ctypes.byref(st)))
+        return rp[0]
+
+    out = {}
+    for name, with_depth in (("colour", False), ("colour_depth", True)):
+        ctx.present_draw(variants[0], mh, with_depth=with_depth)  # warm-up: buffers
+        wait()
+        t0 = time.perf_counter()
+        ctx.present_draw(variants[0], mh, with_depth=with_depth)
+        for k in range(1, n):
+            ctx.present_draw(variants[k % len(variants)], mh, with_depth=with_depth)
+            wait()
+        wait()
+        out[name] = round(n / (time.perf_counter() - t0), 1)
+    out["bytes_per_frame"] = {"colour": W * H * 4, "colour_depth": W * H * 8}
+    return out
+
+
 def pct(xs, p):
     xs = sorted(xs)
     k = min(len(xs) - 1, max(0, int(math.ceil(p / 100.0 * len(xs))) - 1))
@@ -548,6 +580,7 @@ def main():
             t0 = time.perf_counter()
             ctx.draw(variants[k % n_variants], mh)
             lat_host.append((time.perf_counter() - t0) * 1e3)
+    present = None if args.profile else present_rates(ctx, variants, mh, W, H)
 
     batched = [k for k in ("graph", "stream") if k in fps]
     main_form = max(batched, key=lambda k: fps[k]) if batched else "plain"
@@ -640,6 +673,9 @@ def main():
                         "device_p99": round(pct(lat_dev, 99), 4),
                         "host_p50": round(pct(lat_host, 50), 4),
                         "host_p99": round(pct(lat_host, 99), 4)} if lat_dev else None),
+        # Frames delivered into host memory by the present pair (two in
+        # flight): the PCIe-inclusive rate of a window loop, never `value`.
+        "present_frames_per_s": present,
         "frame_stats": stats,
         "compulsory_bytes_per_frame": B_frame,
         "frame_roofline": {"achieved_GBs": round(B_frame * main_fps / 1e9, 2),
