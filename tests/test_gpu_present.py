@@ -26,8 +26,9 @@ def _check(scene, got, want):
 
 
 def test_present_two_frames_in_flight(ctx):
+    # 77x53: the pixel copies take the copy kernel's unaligned path.
     scenes = [S.config1("ref"), C.sphere(), S.config1("cfg"), C.cube_scene(),
-              C.sphere(w=200, h=120, eye=(0, 0, 1.3))]
+              C.sphere(w=200, h=120, eye=(0, 0, 1.3)), C.sphere(w=77, h=53)]
     handles = [ctx.upload_scene(s)[0] for s in scenes]
     wants = [refcpu.draw(s) for s in scenes]
     with pytest.raises(_lib.GorenderError):
