@@ -68,11 +68,11 @@ def test_mix_equals_alone(ctx, resident, budget_mib, max_batch, max_inflight):
 
 def test_all_benchmark_requests(ctx, resident_all):
     """The 64 distinct requests of the serving benchmark, served at its
-    settings (2 GiB budget, batches of up to 16, 2 launches in flight), each
+    settings (2 GiB budget, batches of up to 16, 3 launches in flight), each
     equal to the oracle's digest of that request drawn alone."""
     rs, reqs = resident_all
     assert len(reqs) == 64
-    with Server(ctx, 2 << 30, 16, 2) as srv:
+    with Server(ctx, 2 << 30, 16, 3) as srv:
         tickets = [srv.submit(r) for r in reqs]
         results = [srv.wait(t) for t in tickets]
     assert max(r.batch_size for r in results) > 1
@@ -230,3 +230,28 @@ def test_deadline_sheds_late_requests(ctx, resident):
                 shed += 1
     print(f"{done} served, {shed} shed", flush=True)
     assert shed > 0 and done > 0
+
+
+@pytest.mark.parametrize("colour_only", [False, True])
+def test_odd_frame_sizes(ctx, colour_only):
+    """Frames whose pixel bytes are not a multiple of 16 (the readback copy
+    kernel's unaligned path), batched together, equal the oracle bit for bit;
+    colour_only returns no depth."""
+    from oracle import refcpu
+    from scene_cases import cube_scene, sphere
+    scenes = [sphere(w=77, h=53), cube_scene(w=45, h=31), sphere(w=200, h=120, eye=(0, 0, 1.4))]
+    reqs = [PreparedRequest(sc, ctx.upload_scene(sc)[0]) for sc in scenes]
+    with Server(ctx, 64 << 20, 8, 3, colour_only=colour_only) as srv:
+        tickets = [srv.submit(r) for r in reqs]
+        results = [srv.wait(t) for t in tickets]
+    for sc, res in zip(scenes, results):
+        want = refcpu.draw(sc)
+        h, w = sc.frame.height, sc.frame.width
+        assert np.array_equal(res.rgba.reshape(h, w, 4), want[0]), sc.name
+        if colour_only:
+            assert res.depth is None
+        else:
+            assert np.array_equal(res.depth.reshape(h, w).view(np.uint32),
+                                  want[1].view(np.uint32)), sc.name
+        assert res.stats["tpf"] == want[2]["tpf"], sc.name
+
