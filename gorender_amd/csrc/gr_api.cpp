@@ -76,7 +76,9 @@ struct Pipe {
     DevBuf items;  // raster work items (later parts of split tiles)
     DevBuf slabs;  // depth keys of split tiles; zero between draws
     DevBuf okey;   // overlay keys of the frames that draw overlays; zero between draws
-    DevBuf orec;   // overlay records per slot (batches with overlay frames)
+    DevBuf orec;   // overlay records per triangle record (batches with overlay frames)
+    DevBuf rbase;  // record pool counters (zero between draws), then each face block's first
+                   // overflow record
     hipEvent_t raster_done = nullptr;  // after this pipe's last k_raster
     bool raster_pending = false;
 };
@@ -116,6 +118,7 @@ struct gr_context {
     // Live batch graphs (their control words are checked by gr_synchronize).
     std::vector<gr_batch_graph*> graphs;
     const uint32_t* last_overflow = nullptr;  // device overflow flags of the last draw's frames
+    const gr::DevBatchCounters* last_counters = nullptr;  // device counters of the last enqueue
 
     // The present path (gr_present_draw / gr_present_wait): two frames in
     // flight, each drawn into device buffers and copied to pinned host memory.
@@ -489,10 +492,12 @@ gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_ob
             seq += o.face_blocks;
         }
         d.face_blocks = P.fblocks - d.face_block_begin;
-        d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)(P.slots * sizeof(gr::OverlayRec) + 1) : nullptr;
-        d.rrec = (gr::RasterRec*)(uintptr_t)(P.slots * sizeof(gr::RasterRec));
-        d.srec = (gr::ShadeRec*)(uintptr_t)(P.slots * sizeof(gr::ShadeRec));
-        d.wrec = (gr::WideRec*)(uintptr_t)(P.slots * sizeof(gr::WideRec));
+        // Set to the batch's record arrays by write_header (orec: 1 = overlays).
+        d.orec = d.okey ? (gr::OverlayRec*)(uintptr_t)1 : nullptr;
+        d.rrec = nullptr;
+        d.srec = nullptr;
+        d.wrec = nullptr;
+        d.rbase = nullptr;
         P.slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
     }
     P.nobj = nobj;
@@ -514,13 +519,23 @@ gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_ob
     return GR_OK;
 }
 
+// Words of the record-pool counters.
+constexpr size_t kPoolWords = (size_t)gr::kRecPools * gr::kPoolStride;
+
 // Device buffers of one batch: a pipe's, a batch graph's, or carved from the
 // fixed scratch array of the no-allocation draw.
 struct Bufs {
     char* header = nullptr;  // frames | objects | stats | counters | frame_overflow | XCD tables
+    // Triangle records [face blocks * kRecsPerBlock + rec_pools * pool_recs]:
+    // each face block's own, then the overflow pools, claimed by the blocks
+    // that need more as they run; rbase [face blocks] holds each block's first
+    // overflow record, rec_pool the pool counters (zero between draws).
     gr::RasterRec* rrec = nullptr;
     gr::ShadeRec* srec = nullptr;
     gr::WideRec* wrec = nullptr;
+    uint32_t rec_pools = 1, pool_recs = 0;
+    uint32_t* rbase = nullptr;
+    uint32_t* rec_pool = nullptr;
     uint32_t* tile_counts = nullptr;  // zero between draws (k_raster re-zeroes)
     uint32_t* tiles = nullptr;        // offsets | cursors | slabs
     gr::RasterItem* items = nullptr;
@@ -556,11 +571,12 @@ void write_header(const Plan& P, const Bufs& B, void* h) {
     for (int f = 0; f < P.nf; f++) {
         gr::DevFrame& d = hf[f];
         if (d.okey) d.okey = (uint32_t*)((char*)B.okey + ((uintptr_t)d.okey - 1));
-        if (d.orec) d.orec = (gr::OverlayRec*)((char*)B.orec + ((uintptr_t)d.orec - 1));
+        if (d.orec) d.orec = B.orec;
         d.ctrl = B.ctrl;
-        d.rrec = (gr::RasterRec*)((char*)B.rrec + (uintptr_t)d.rrec);
-        d.srec = (gr::ShadeRec*)((char*)B.srec + (uintptr_t)d.srec);
-        d.wrec = (gr::WideRec*)((char*)B.wrec + (uintptr_t)d.wrec);
+        d.rrec = B.rrec;
+        d.srec = B.srec;
+        d.wrec = B.wrec;
+        d.rbase = B.rbase + d.face_block_begin;
     }
     if (P.nobj) std::memcpy((char*)h + P.off_objs, P.dobj, (size_t)P.nobj * sizeof(gr::DevObject));
     std::memcpy((char*)h + P.off_xcd, P.xcd, xcd_ints(P.nf) * sizeof(int));
@@ -589,6 +605,11 @@ gr::LaunchParams launch_params(const Plan& P, const Bufs& B) {
     p.part_refs = B.part_refs;
     p.frame_overflow = (uint32_t*)(bb + P.off_over);
     p.wg_count = B.wgcount;
+    p.rec_base = B.rbase;
+    p.rec_pool = B.rec_pool;
+    p.rec_pools = B.rec_pools;
+    p.pool_recs = B.pool_recs;
+    p.ovf_begin = (uint32_t)((size_t)P.fblocks * gr::kRecsPerBlock);
     p.refs = B.refs;
     p.refs_capacity = B.refs_cap;
     p.stats = (unsigned long long*)(bb + P.off_stats);
@@ -616,8 +637,9 @@ gr_status launch_plan(gr_context* c, const Plan& P, const gr::LaunchParams& p, h
     if (ok && P.decorate) timed_launch(c, "k_decorate", s, [&] { gr::launch_decorate(p, s); });
     hipError_t le = hipGetLastError();
     if (!ok || le != hipSuccess) {
-        // Leave the tile counters zeroed for the next draw.
+        // Leave the tile and record-pool counters zeroed for the next draw.
         gr::launch_fill32(p.tile_counts, 0u, (size_t)P.tiles, s);
+        gr::launch_fill32(p.rec_pool, 0u, kPoolWords, s);
         if (!ok) return fail(GR_ERR_INVALID_ARGUMENT, "unsupported tile size");
         return fail(GR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(le));
     }
@@ -634,15 +656,36 @@ gr_status ensure_zeroed(gr_context* c, DevBuf& b, size_t bytes, hipStream_t s) {
     return GR_OK;
 }
 
+// Overflow pools of a batch: one per face block up to kRecPools, together one
+// record per kOvfRefs tile-list entries of the capacity (grown on demand for
+// both) and each at most what its blocks can claim beyond their own ranges.
+void record_pools(const Plan& P, unsigned long long refs_cap, uint32_t* pools, uint32_t* per) {
+    const unsigned long long np = (unsigned long long)std::min(gr::kRecPools, std::max(P.fblocks, 1));
+    const unsigned long long most = ((unsigned long long)std::max(P.fblocks, 1) + np - 1) / np *
+                                    (unsigned long long)(gr::kSlotsPerWg - gr::kRecsPerBlock);
+    *pools = (uint32_t)np;
+    *per = (uint32_t)std::max<unsigned long long>(1, std::min(refs_cap / gr::kOvfRefs / np, most));
+}
+
+// Records of a batch: each face block's own, then the overflow pools.
+size_t records_of(const Plan& P, uint32_t pools, uint32_t per) {
+    return (size_t)P.fblocks * gr::kRecsPerBlock + (size_t)pools * per;
+}
+
 // Device buffers of a plan in a set of growable buffers (a pipe's or a batch
 // graph's); new allocations that must start at zero are zero-filled on s.
 gr_status size_bufs(gr_context* c, const Plan& P, Pipe& Q, unsigned long long refs_cap,
                     uint32_t part_refs, int TW, int TH, hipStream_t s, Bufs& B) {
     gr_status st;
+    uint32_t pools, per;
+    record_pools(P, refs_cap, &pools, &per);
+    const size_t recs = records_of(P, pools, per);
     if ((st = ensure(c, Q.batch, P.header_bytes))) return st;
-    if ((st = ensure(c, Q.rrec, P.slots * sizeof(gr::RasterRec)))) return st;
-    if ((st = ensure(c, Q.srec, P.slots * sizeof(gr::ShadeRec)))) return st;
-    if ((st = ensure(c, Q.wrec, P.slots * sizeof(gr::WideRec)))) return st;
+    if ((st = ensure(c, Q.rrec, recs * sizeof(gr::RasterRec)))) return st;
+    if ((st = ensure(c, Q.srec, recs * sizeof(gr::ShadeRec)))) return st;
+    if ((st = ensure(c, Q.wrec, recs * sizeof(gr::WideRec)))) return st;
+    if ((st = ensure_zeroed(c, Q.rbase, (kPoolWords + std::max(P.fblocks, 1)) * sizeof(uint32_t), s)))
+        return st;
     if ((st = ensure_zeroed(c, Q.tile_counts, (size_t)P.tiles * 4, s))) return st;
     if ((st = ensure(c, Q.tiles, 3 * (size_t)P.tiles * sizeof(uint32_t)))) return st;
     const size_t slab_bytes = (size_t)TW * TH * 8;
@@ -654,13 +697,17 @@ gr_status size_bufs(gr_context* c, const Plan& P, Pipe& Q, unsigned long long re
     if ((st = ensure(c, Q.refs, refs_cap * sizeof(uint32_t)))) return st;
     if ((st = ensure_zeroed(c, Q.ctrl, sizeof(gr::DevControl), s))) return st;
     if (P.okey_words) {
-        if ((st = ensure(c, Q.orec, P.slots * sizeof(gr::OverlayRec)))) return st;
+        if ((st = ensure(c, Q.orec, recs * sizeof(gr::OverlayRec)))) return st;
         if ((st = ensure_zeroed(c, Q.okey, P.okey_words * 4, s))) return st;
     }
     B.header = (char*)Q.batch.p;
     B.rrec = (gr::RasterRec*)Q.rrec.p;
     B.srec = (gr::ShadeRec*)Q.srec.p;
     B.wrec = (gr::WideRec*)Q.wrec.p;
+    B.rec_pools = pools;
+    B.pool_recs = per;
+    B.rec_pool = (uint32_t*)Q.rbase.p;
+    B.rbase = B.rec_pool + kPoolWords;
     B.tile_counts = (uint32_t*)Q.tile_counts.p;
     B.tiles = (uint32_t*)Q.tiles.p;
     B.items = (gr::RasterItem*)Q.items.p;
@@ -744,6 +791,7 @@ gr_status enqueue(gr_context* c, const gr_batch_frame* bf, int nf, const gr_obje
     c->last_frames = nf;
     c->last_stats = p.stats;
     c->last_overflow = p.frame_overflow;
+    c->last_counters = p.counters;
     return GR_OK;
 }
 
@@ -812,7 +860,8 @@ gr_status sync_and_check(gr_context* c, unsigned long long* needed) {
     if (unsupported) return fail(GR_ERR_UNSUPPORTED, "overlay line longer than 65536 pixels");
     if (noalloc_overflow && !overflow)
         return fail(GR_ERR_CAPACITY,
-                    "tile lists of a no-allocation draw exceeded its fixed scratch array");
+                    "tile lists or triangle records of a no-allocation draw exceeded its "
+                    "fixed scratch array");
     if (overflow) return GR_ERR_CAPACITY;
     return GR_OK;
 }
@@ -871,7 +920,7 @@ gr_status present_enqueue(gr_context* c, gr_context::PresentSlot& ps) {
 void free_pipe_bufs(Pipe& pp) {
     DevBuf* pb[] = {&pp.batch, &pp.rrec,    &pp.srec, &pp.wrec,  &pp.tile_counts,
                     &pp.tiles, &pp.wgcount, &pp.refs, &pp.ctrl,  &pp.items,
-                    &pp.slabs, &pp.okey,    &pp.orec};
+                    &pp.slabs, &pp.okey,    &pp.orec,  &pp.rbase};
     for (DevBuf* b : pb) {
         if (b->p) (void)hipFree(b->p);
         *b = DevBuf{};
@@ -1096,9 +1145,11 @@ gr_status check_request(gr_context* c, const gr_frame* frame, const gr_object* o
 }
 
 // The no-allocation draw. Layout of [base, base + bytes): control words (sticky
-// until gr_synchronize reads them), batch header, texture table, records, tile
-// counters, tile offsets/cursors/slabs, face-block counts, then tile-list
-// entries in what remains. Tiles are never split (no slabs or work items).
+// until gr_synchronize reads them), batch header, texture table, tile counters,
+// tile offsets/cursors/slabs, face-block counts and first records, then what
+// remains shared between triangle records and tile-list entries, as many of
+// each (fewer records when the frame has fewer slots). Tiles are never split
+// (no slabs or work items).
 // Host data reaches the array through kernel arguments (k_put) and zeroing is
 // a fill kernel, so nothing here allocates memory of any kind, and the launch
 // sequence holds no memset or host-buffer copy.
@@ -1136,18 +1187,32 @@ gr_status enqueue_noalloc(gr_context* c, const gr_frame* frame, const gr_object*
         B.ctrl = (gr::DevControl*)take(sizeof(gr::DevControl));
         B.header = take(P.header_bytes);
         B.textures = (const gr::DevTexture*)take(ntex * sizeof(gr::DevTexture));
-        B.rrec = (gr::RasterRec*)take(P.slots * sizeof(gr::RasterRec));
-        B.srec = (gr::ShadeRec*)take(P.slots * sizeof(gr::ShadeRec));
-        B.wrec = (gr::WideRec*)take(P.slots * sizeof(gr::WideRec));
         B.tile_counts = (uint32_t*)take((size_t)P.tiles * 4);
         B.tiles = (uint32_t*)take(3 * (size_t)P.tiles * 4);
         B.wgcount = (uint32_t*)take((size_t)std::max(P.fblocks, 1) * 4);
+        B.rec_pool = (uint32_t*)take(kPoolWords * 4);
+        B.rbase = (uint32_t*)take((size_t)std::max(P.fblocks, 1) * 4);
+        constexpr size_t kRecBytes =
+            sizeof(gr::RasterRec) + sizeof(gr::ShadeRec) + sizeof(gr::WideRec);
         const size_t min_refs = 1u << 16;
-        if (off + min_refs * 4 > bytes)
+        // Every block's own records, at least one overflow record per pool and
+        // 2^16 tile-list entries; three record arrays and the entries each
+        // start on a 256-byte boundary.
+        const size_t own = (size_t)P.fblocks * gr::kRecsPerBlock;
+        const size_t need = off + 4 * 256 + min_refs * 4 + (own + gr::kRecPools) * kRecBytes;
+        if (need > bytes)
             return ::fail(GR_ERR_INVALID_ARGUMENT,
                           "frame too large for the no-allocation draw: needs " +
-                              std::to_string((off + min_refs * 4) >> 20) +
+                              std::to_string(need >> 20) +
                               " MiB of its 32 MiB device array (use gr_draw_device)");
+        // The rest: kOvfRefs tile-list entries per overflow record.
+        const size_t rest = bytes - off - 4 * 256 - own * kRecBytes;
+        record_pools(P, rest / (kRecBytes + 4 * gr::kOvfRefs) * gr::kOvfRefs, &B.rec_pools,
+                     &B.pool_recs);
+        const size_t recs = records_of(P, B.rec_pools, B.pool_recs);
+        B.rrec = (gr::RasterRec*)take(recs * sizeof(gr::RasterRec));
+        B.srec = (gr::ShadeRec*)take(recs * sizeof(gr::ShadeRec));
+        B.wrec = (gr::WideRec*)take(recs * sizeof(gr::WideRec));
         B.refs = (uint32_t*)((char*)base + off);
         B.refs_cap = std::min<unsigned long long>((bytes - off) / 4, (unsigned long long)kMaxRefs);
         B.part_refs = 0xFFFFFFFFu;  // no split tiles
@@ -1179,8 +1244,9 @@ gr_status enqueue_noalloc(gr_context* c, const gr_frame* frame, const gr_object*
                                n * sizeof(gr::DevTexture), s);
             }
         }
-        // Tile counters start at zero (a failed earlier draw may have left some).
+        // Tile and record-pool counters start at zero (the array is shared).
         gr::launch_fill32(B.tile_counts, 0u, (size_t)P.tiles, s);
+        gr::launch_fill32(B.rec_pool, 0u, kPoolWords, s);
         GR_HIP(hipGetLastError());
         const gr::LaunchParams p = launch_params(P, B);
         if ((st = launch_plan(c, P, p, s, after))) return st;
@@ -2095,6 +2161,10 @@ gr_status gr_autotune(gr_context* c, const gr_batch_frame* frames, int32_t nf,
                     if ((st = read_stats(c, sv))) return st;
                     unsigned long long refs = 0;
                     for (int f = 0; f < nf; f++) refs += sv[(size_t)f * gr::kNumStats + gr::kStatRefs];
+                    // The capacity also holds the triangle records the face blocks claimed.
+                    gr::DevBatchCounters bc;
+                    GR_HIP(hipMemcpy(&bc, c->last_counters, sizeof bc, hipMemcpyDeviceToHost));
+                    refs = std::max(refs, bc.recs_used);
                     c->cfg.refs_capacity = std::max<int64_t>(1 << 16, (int64_t)grown(refs));
                     GR_HIP(hipEventRecord(ea, c->stream));
                     for (int r = 0; r < reps; r++)

@@ -15,6 +15,20 @@ constexpr int kFacesPerWg = 256;
 constexpr int kMaxFan = 7;
 constexpr int kSlotsPerWg = kFacesPerWg * kMaxFan;  // 1792
 constexpr int kMaxClip = 9;                         // maxClipPoints (clipping.go:10)
+
+// Triangle records: face block b (of the batch) owns records
+// [b * kRecsPerBlock, (b + 1) * kRecsPerBlock), enough for every face of the
+// block unclipped plus 64 more records. A block that may need more (it
+// reserves kMaxFan records for each face that reaches the full clipper)
+// claims the rest from the overflow records, split into kRecPools pools
+// (block b from pool b % pools), each with its own counter kPoolStride words
+// from the next so that the claims of a launch do not serialise on one
+// address. The overflow records number one per kOvfRefs tile-list entries of
+// the batch's capacity, which grows on demand.
+constexpr int kRecsPerBlock = kFacesPerWg + 64;  // 320
+constexpr int kRecPools = 64;
+constexpr int kPoolStride = 64;
+constexpr int kOvfRefs = 16;
 constexpr int kMaxDim = 16384;                      // framebuffer side limit
 
 // Coordinates within +-kFastCoord keep every edge value of a pixel inside a
@@ -96,11 +110,16 @@ struct DevFrame {
     uint8_t* rgba;               // device outputs
     float* depth;
     uint32_t* okey;              // overlay keys [H*W] or null: (slot+1) << 1 | vertex
-    struct OverlayRec* orec;     // per slot when okey is set
+    struct OverlayRec* orec;     // per triangle record when okey is set
     DevControl* ctrl;
-    struct RasterRec* rrec;      // frame-local slot arrays
+    // Triangle records of the batch. A slot (the draw-order number of a
+    // triangle: frame-local face block * kSlotsPerWg + its offset in the
+    // block) has its record in the block's own range (offsets below
+    // kRecsPerBlock) or at rbase[block] + offset - kRecsPerBlock.
+    struct RasterRec* rrec;
     struct ShadeRec* srec;
     struct WideRec* wrec;
+    const uint32_t* rbase;       // first overflow record of each face block of this frame
 };
 
 // Per-triangle raster set-up, 80 bytes: what the visibility pass reads. The
@@ -159,7 +178,8 @@ struct DevBatchCounters {
     uint32_t n_items;              // raster work items (later parts of split tiles)
     uint32_t next_item;            // next work item to take
     uint32_t n_slabs;              // slabs handed to split tiles
-    uint32_t pad[3];
+    uint32_t pad;
+    unsigned long long recs_used;  // triangle records the face blocks claimed (k_bin_scan)
 };
 
 // Tile not split (no slab); also the end of the raster work queue.
@@ -171,9 +191,10 @@ struct RasterItem {
 };
 
 // Sticky control words (cleared by the host when it reads them): set when a
-// frame of any batch needed more tile-list entries than were allocated.
+// frame of any batch needed more tile-list entries or triangle records than
+// were allocated (one capacity sizes both).
 struct DevControl {
-    unsigned long long refs_needed;
+    unsigned long long refs_needed;  // the capacity that would have sufficed
     uint32_t overflow;
     uint32_t unsupported;  // an overlay line longer than kMaxOverlayLine
 };

@@ -540,11 +540,11 @@ __device__ __forceinline__ void count_tiles(const DevFrame& fr, const TileTally&
         }
 }
 
-// Writes the set-up of one post-clip triangle into its slot (renderer.go:361-394
+// Writes the set-up of one post-clip triangle into record ri (renderer.go:361-394
 // then the per-triangle part of rasterizer.go:98-137). Returns the TPF
 // references in the low 16 bits and 1 << 16 when the triangle was binned.
 __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, const CV& c,
-                                  int tex, uint32_t slot, const TileTally& tally, int twl,
+                                  int tex, uint32_t ri, const TileTally& tally, int twl,
                                   int thl) {
     uint32_t tpf = 0, tris = 0;
     const CV vv[3] = {a, b, c};
@@ -609,7 +609,7 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
         }
         o.binned = binned ? 1u : 0u;
         o.pad = 0;
-        fr.orec[slot] = o;
+        fr.orec[ri] = o;
     }
     if (binned) {
         tris = 1;
@@ -689,7 +689,7 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                 }
                 wr.adj = adj;
                 wr.pad[0] = wr.pad[1] = wr.pad[2] = 0;
-                fr.wrec[slot] = wr;
+                fr.wrec[ri] = wr;
             }
             ShadeRec sr;
             // u, v as clipped; finalize_record divides them by z (rasterizer.go:132-137).
@@ -704,18 +704,19 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
             sr.in[2] = c.i;
             sr.tex = tex;
             sr.pad[0] = sr.pad[1] = 0;
-            fr.srec[slot] = sr;
+            fr.srec[ri] = sr;
         }
     }
-    fr.rrec[slot] = rr;
+    fr.rrec[ri] = rr;
     if (rr.bx0 <= rr.bx1)
         count_tiles(fr, tally, (int)rr.bx0 >> twl, (int)rr.bx1 >> twl, (int)rr.by0 >> thl,
                     (int)rr.by1 >> thl);
     return tpf | (tris << 16);
 }
 
-// A slot that holds no triangle: an empty box, never binned or finalized.
-__device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t slot) {
+// A slot that holds no triangle (record ri): an empty box, never binned or
+// finalized.
+__device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t ri) {
     RasterRec rr;
     rr.sf = rr.z0 = rr.z1 = rr.z2 = 0.f;
     rr.rsf = rr.rz0 = rr.rz1 = rr.rz2 = 0.f;
@@ -726,11 +727,27 @@ __device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t slot) 
     rr.wide = 0;
 #pragma unroll
     for (int e = 0; e < 9; e++) rr.e[e] = 0;
-    fr.rrec[slot] = rr;
+    fr.rrec[ri] = rr;
     if (fr.orec) {
         OverlayRec o = {};
-        fr.orec[slot] = o;
+        fr.orec[ri] = o;
     }
+}
+
+// Record of offset `off` of face block gb (of the batch) whose overflow
+// records start at ovf.
+__device__ __forceinline__ uint32_t block_rec(uint32_t gb, uint32_t ovf, uint32_t off) {
+    return off < (uint32_t)kRecsPerBlock ? gb * (uint32_t)kRecsPerBlock + off
+                                         : ovf + (off - (uint32_t)kRecsPerBlock);
+}
+
+// Record of slot `slot` of a frame (the block's overflow start is read only
+// past its own range).
+__device__ __forceinline__ uint32_t rec_index(const DevFrame& fr, uint32_t slot) {
+    const uint32_t blk = slot / (uint32_t)kSlotsPerWg, off = slot % (uint32_t)kSlotsPerWg;
+    return off < (uint32_t)kRecsPerBlock
+               ? ((uint32_t)fr.face_block_begin + blk) * (uint32_t)kRecsPerBlock + off
+               : fr.rbase[blk] + (off - (uint32_t)kRecsPerBlock);
 }
 
 // General faces clipped per round by one wave; each fills kMaxFan lanes.
@@ -771,10 +788,11 @@ __device__ __forceinline__ TileRect rec_tiles(const RasterRec& r, int twl, int t
     return t;
 }
 
-// Union of the tile ranges of the workgroup's records [slot0, slot0 + cnt),
-// into lrect (LDS, 4 ints) by LDS atomics. Ends with a barrier.
-__device__ void block_tile_rect(const RasterRec* rrec, uint32_t slot0, uint32_t cnt, int twl,
-                                int thl, int* lrect) {
+// Union of the tile ranges of the records of offsets [0, cnt) of face block gb
+// (overflow records from ovf), into lrect (LDS, 4 ints) by LDS atomics. Ends
+// with a barrier.
+__device__ void block_tile_rect(const RasterRec* rr, uint32_t gb, uint32_t ovf, uint32_t cnt,
+                                int twl, int thl, int* lrect) {
     if (threadIdx.x == 0) {
         lrect[0] = 0x7FFFFFFF;
         lrect[1] = -1;
@@ -784,7 +802,7 @@ __device__ void block_tile_rect(const RasterRec* rrec, uint32_t slot0, uint32_t 
     __syncthreads();
     int a = 0x7FFFFFFF, b = -1, c = 0x7FFFFFFF, d = -1;
     for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-        TileRect t = rec_tiles(rrec[slot0 + i], twl, thl);
+        TileRect t = rec_tiles(rr[block_rec(gb, ovf, i)], twl, thl);
         if (t.tx0 > t.tx1) continue;
         a = min(a, t.tx0);
         b = max(b, t.tx1);
@@ -899,13 +917,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                               uint32_t* __restrict__ wg_count,
                                               unsigned long long* __restrict__ stats, int twl,
                                               int thl, const int* __restrict__ xcd_face,
-                                              int nframes) {
+                                              int nframes, uint32_t* __restrict__ rec_base,
+                                              uint32_t* __restrict__ rec_pool, uint32_t npools,
+                                              uint32_t pool_recs, uint32_t ovf_begin,
+                                              DevControl* __restrict__ ctrl,
+                                              uint32_t* __restrict__ frame_overflow) {
     static_assert(kFacesPerWg == 64 * kFacesPerLane, "whole faces per lane");
     // Survivors in face order: face offset | first clip plane << 8.
     __shared__ uint32_t lds_sface[kFacesPerWg];
     __shared__ CV wpoly[2][kMaxClip];
     __shared__ CV lds_stage[kStage][kMaxClip];  // clipped polygons of general faces
-    __shared__ int lds_stage_k[kStage], lds_stage_tex[kStage], lds_stage_slot[kStage];
+    __shared__ int lds_stage_k[kStage], lds_stage_tex[kStage];
+    __shared__ int lds_stage_off[kStage];  // the face's first offset in the block
     __shared__ uint32_t lds_tile_tag[kTileTable], lds_tile_cnt[kTileTable];
 
     int bf, blocal;
@@ -935,7 +958,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // backface test and plane classification. The survivors (front faces the
     // clipper does not reject outright) are compacted in face order into LDS,
     // so the lighting, clipping and set-up below run on dense lanes.
-    int nsurv = 0;  // wave-uniform
+    int nsurv = 0;      // wave-uniform
+    uint32_t nrec = 0;  // records the survivors can need: 1, or kMaxFan when clipped
     if (vis != kBoxOutside) {
         // All the lane's vertex indices, then all its corners, are loaded
         // before any is used: lanes past the mesh's last face read its last
@@ -980,16 +1004,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             if (survivor)
                 lds_sface[nsurv + __popcll(sm & lanemask_lt(lane))] = (uint32_t)fo | ((uint32_t)p << 8);
             nsurv += __popcll(sm);
+            nrec += (uint32_t)__popcll(sm) + (kMaxFan - 1) * (uint32_t)__popcll(__ballot(survivor && p < 6));
         }
     }
     wave_sync();
+
+    // Records beyond the block's own range (sized for the largest fan of every
+    // clipped survivor): a contiguous range of its overflow pool, claimed now.
+    // Past the pool's end the block draws nothing and the batch reports an
+    // overflow with a capacity that gives every pool this much (the caller
+    // grows the capacity and draws it again); its frame is marked so that its
+    // raster draws the background only.
+    uint32_t ovf = 0u;  // wave-uniform
+    if (nsurv > 0 && nrec > (uint32_t)kRecsPerBlock) {
+        const uint32_t extra = nrec - (uint32_t)kRecsPerBlock;
+        const uint32_t pool = (uint32_t)b % npools;
+        uint32_t r = 0u;
+        if (lane == 0) r = atomicAdd(&rec_pool[pool * kPoolStride], extra);
+        r = (uint32_t)__builtin_amdgcn_readfirstlane((int)r);
+        const unsigned long long end = (unsigned long long)r + extra;
+        if (end > pool_recs) {
+            if (lane == 0) {
+                atomicMax(&ctrl->refs_needed, end * npools * kOvfRefs);
+                atomicOr(&ctrl->overflow, 1u);
+                atomicOr(&frame_overflow[o.frame], 1u);
+            }
+            nsurv = 0;
+        } else {
+            ovf = ovf_begin + pool * pool_recs + r;
+        }
+    }
+    if (lane == 0) rec_base[b] = ovf;
 
     // Pass 2, survivor r0 + lane, 64 a round: lighting, texture coordinates,
     // clipping. 0: nothing, 1: emit the triangle as is, 2: full clipper in the
     // wave, 3: one-plane polygon. q[0..nq-1] holds the polygon to emit (the
     // triangle itself for modes 1 and 2), so only one set of corners stays
     // live across the emits.
-    const uint32_t wg_slot0 = (uint32_t)(o.seq_block + wg_in_obj) * kSlotsPerWg;
     uint32_t run = 0;  // slots of the earlier rounds (wave-uniform)
     for (int r0 = 0; r0 < nsurv; r0 += 64) {
         int mode = 0;
@@ -1080,16 +1131,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         if (mode == 2) n = kMaxFan;
         const uint64_t b1 = __ballot(n == 1), b2 = __ballot(n == 2), b7 = __ballot(n == kMaxFan);
         const uint64_t lt = lanemask_lt(lane);
-        const uint32_t slot0 = wg_slot0 + run +
-                               (uint32_t)(__popcll(b1 & lt) + 2 * __popcll(b2 & lt) +
-                                          kMaxFan * __popcll(b7 & lt));
+        // The lane's slots start at offset off of the block's.
+        const uint32_t off = run + (uint32_t)(__popcll(b1 & lt) + 2 * __popcll(b2 & lt) +
+                                              kMaxFan * __popcll(b7 & lt));
         run += (uint32_t)(__popcll(b1) + 2 * __popcll(b2) + kMaxFan * __popcll(b7));
         // Fan of q (Polygon.Triangulate, clipping.go:44-62): (P0, P1, P2),
         // then (P0, P2, P3) for a quad.
         if (mode == 1 || mode == 3)
-            add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, slot0, tally, twl, thl));
+            add_emit(st, emit_triangle(fr, q[0], q[1], q[2], tex, block_rec(b, ovf, off), tally,
+                                       twl, thl));
         if (mode == 3 && nq == 4)
-            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, slot0 + 1, tally, twl, thl));
+            add_emit(st, emit_triangle(fr, q[0], q[2], q[3], tex, block_rec(b, ovf, off + 1),
+                                       tally, twl, thl));
 
         // General faces, kStage at a time: the wave clips each one into its
         // stage (clipping.go:167-236), then one lane per fan slot emits the
@@ -1103,27 +1156,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 const int L = __builtin_ctzll(gen);
                 const CV a = shfl_cv(q[0], L), bb = shfl_cv(q[1], L), c = shfl_cv(q[2], L);
                 const int texL = __shfl(tex, L, 64);
-                const int sL = __shfl((int)slot0, L, 64);
+                const int sL = __shfl((int)off, L, 64);
                 const int k = wave_clip(buf0, buf1, a, bb, c, fr.znear, fr.zfar, lane);
                 if (k < 0 && lane == L) st.packed += 1u << 28;
                 if (lane < k) lds_stage[nst][lane] = buf0[lane];
                 if (lane == 0) {
                     lds_stage_k[nst] = k;
                     lds_stage_tex[nst] = texL;
-                    lds_stage_slot[nst] = sL;
+                    lds_stage_off[nst] = sL;
                 }
                 wave_sync();
             }
             const int r = lane / kMaxFan, t = lane - r * kMaxFan;
             if (r < nst) {
                 const int k = lds_stage_k[r];
-                const uint32_t slot = (uint32_t)lds_stage_slot[r] + (uint32_t)t;
+                const uint32_t ri = block_rec(b, ovf, (uint32_t)lds_stage_off[r] + (uint32_t)t);
                 const CV* P = lds_stage[r];
                 if (t < k - 2)
                     add_emit(st, emit_triangle(fr, P[0], P[t + 1], P[t + 2], lds_stage_tex[r],
-                                               slot, tally, twl, thl));
+                                               ri, tally, twl, thl));
                 else
-                    empty_record(fr, slot);
+                    empty_record(fr, ri);
             }
             wave_sync();
         }
@@ -1165,9 +1218,11 @@ __global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ o
     const DevFrame& fr = frames[o.frame];
     if (!fr.orec) return;
     const uint32_t cnt = wg_count[b];
-    const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
+    const uint32_t blk = (uint32_t)(o.seq_block + (b - o.face_block));  // in the frame
+    const uint32_t slot0 = blk * kSlotsPerWg;
+    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? fr.rbase[blk] : 0u;
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
-        const OverlayRec rec = fr.orec[slot0 + i];
+        const OverlayRec rec = fr.orec[block_rec(b, ovf, i)];
         // A frame with a line too long is counted on its own, besides the
         // batch's control word, so a caller can tell which frames failed.
         if (rec.binned && !draw_overlays(fr, rec, slot0 + i))
@@ -1279,7 +1334,8 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
                                                    uint32_t* __restrict__ tile_slab,
                                                    RasterItem* __restrict__ items,
                                                    uint32_t max_items, uint32_t max_slabs,
-                                                   uint32_t part_refs) {
+                                                   uint32_t part_refs,
+                                                   uint32_t* __restrict__ rec_pool) {
     __shared__ unsigned long long wsum[16];
     __shared__ unsigned long long base_s;
     const int f = blockIdx.x;
@@ -1287,6 +1343,17 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
     const int ntiles = fr.tiles_x * fr.tiles_y;
     uint32_t* cnt = tile_counts + fr.tile_begin;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // k_face is done with the record pools: their total to the batch counters,
+    // and the counters back to zero for the next draw.
+    static_assert(kRecPools % 64 == 0 && kRecPools <= 1024, "whole waves read the pool counters");
+    if (f == 0 && threadIdx.x < kRecPools) {
+        uint32_t* pc = &rec_pool[threadIdx.x * kPoolStride];
+        const uint32_t v = *pc;
+        *pc = 0u;
+        const unsigned long long t = wave_sum_u64(v);
+        if (lane == 0) atomicAdd(&counters->recs_used, t);
+    }
 
     unsigned long long local = 0;
     for (int i = threadIdx.x; i < ntiles; i += 1024) local += cnt[i];
@@ -1373,18 +1440,24 @@ __global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __re
     const DevFrame& fr = frames[o.frame];
     const uint32_t cnt = wg_count[b];
     if (cnt == 0) return;
-    const uint32_t slot0 = (uint32_t)(o.seq_block + (b - o.face_block)) * kSlotsPerWg;
-    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads)
-        finalize_record(&fr.rrec[slot0 + i], &fr.srec[slot0 + i]);
+    // The block's slots start at slot0; its records are block_rec(b, ovf, i).
+    const uint32_t blk = (uint32_t)(o.seq_block + (b - o.face_block));  // in the frame
+    const uint32_t slot0 = blk * kSlotsPerWg;
+    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? fr.rbase[blk] : 0u;
+    const RasterRec* rr = fr.rrec;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
+        const uint32_t ri = block_rec(b, ovf, i);
+        finalize_record(&fr.rrec[ri], &fr.srec[ri]);
+    }
     // The barrier in block_tile_rect orders these writes before the reads below.
-    block_tile_rect(fr.rrec, slot0, cnt, twl, thl, lrect);
+    block_tile_rect(rr, b, ovf, cnt, twl, thl, lrect);
     const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
     if (rx1 < 0) return;
     const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
     if (rw * rh > kLdsTiles) {
         // Very large rectangle: one global atomic per reference.
         for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-            TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+            TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
             for (int ty = t.ty0; ty <= t.ty1; ty++)
                 for (int tx = t.tx0; tx <= t.tx1; tx++) {
                     const uint32_t pos =
@@ -1397,7 +1470,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __re
     for (int i = threadIdx.x; i < rw * rh; i += kFillThreads) lcnt[i] = 0u;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-        TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+        TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++)
                 atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
@@ -1413,7 +1486,7 @@ __global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __re
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-        TileRect t = rec_tiles(fr.rrec[slot0 + i], twl, thl);
+        TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++) {
                 const uint32_t pos = atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
@@ -1645,7 +1718,7 @@ __device__ __forceinline__ unsigned long long raster_list(
         {
             const uint32_t t = threadIdx.x & 63u;
             if (wave < 3 && t < n) {
-                const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[lslot[t]]);
+                const uint4* src = reinterpret_cast<const uint4*>(&fr.rrec[rec_index(fr, lslot[t])]);
                 if (wave > 0) {
                     lrec[t * 5 + wave - 1] = gload(src + wave - 1);
                 } else {
@@ -1678,7 +1751,7 @@ __device__ __forceinline__ unsigned long long raster_list(
             if (info & kWalkWide) {
                 // Coordinates beyond the int32 range: direct int64 evaluation.
                 const uint32_t slot = (uint32_t)sgpr((int)lslot[i]);
-                const WideRec* wr = &fr.wrec[slot];
+                const WideRec* wr = &fr.wrec[rec_index(fr, slot)];
                 const uint4 d4 = lrec[i * 5], r4 = lrec[i * 5 + 1];
                 const float4 rp[2] = {
                     make_float4(__uint_as_float(d4.x), __uint_as_float(d4.y),
@@ -1806,17 +1879,17 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             color = dot ? dot_color : clear_color;
             depth = -1.0f;
         } else {
-            const uint32_t slot = seq - 1u;
-            const RasterRec r = gload(&fr.rrec[slot]);
+            const uint32_t ri = rec_index(fr, seq - 1u);
+            const RasterRec r = gload(&fr.rrec[ri]);
             const float4* rp = reinterpret_cast<const float4*>(&r);
-            const ShadeRec s = gload(&fr.srec[slot]);
+            const ShadeRec s = gload(&fr.srec[ri]);
             float f12f, f20f;
             if (!r.wide) {
                 f12f = (float)(__mul24(r.e[3], x) + __mul24(r.e[4], y) + r.e[5]);
                 f20f = (float)(__mul24(r.e[6], x) + __mul24(r.e[7], y) + r.e[8]);
             } else {
                 int64_t f[3];
-                wide_edges(&fr.wrec[slot], x, y, f);
+                wide_edges(&fr.wrec[ri], x, y, f);
                 f12f = (float)f[1];
                 f20f = (float)f[2];
             }
@@ -2042,7 +2115,8 @@ void launch_face(const LaunchParams& p, hipStream_t s) {
     if (p.face_blocks == 0) return;
     hipLaunchKernelGGL(k_face, dim3(p.face_grid), dim3(64), 0, s, p.objs, p.nobj, p.frames,
                        p.tile_counts, p.wg_count, p.stats, p.tw_log2, p.th_log2, p.xcd_face,
-                       p.nframes);
+                       p.nframes, p.rec_base, p.rec_pool, p.rec_pools, p.pool_recs,
+                       p.ovf_begin, p.ctrl, p.frame_overflow);
 }
 
 void xcd_tables(const int* n, int nframes, int* table, int* grid) {
@@ -2080,7 +2154,7 @@ void launch_bin_scan(const LaunchParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_scan, dim3(p.nframes), dim3(1024), 0, s, p.frames, p.tile_counts,
                        p.tile_offsets, p.tile_cursor, p.frame_overflow, p.stats, p.counters,
                        p.ctrl, p.refs_capacity, p.tile_slab, p.items, p.max_items, p.max_slabs,
-                       p.part_refs);
+                       p.part_refs, p.rec_pool);
 }
 
 void launch_bin_fill(const LaunchParams& p, hipStream_t s) {

@@ -30,6 +30,12 @@ struct LaunchParams {
     uint32_t part_refs;      // tile-list entries per raster part
     uint32_t* frame_overflow;// [nframes]
     uint32_t* wg_count;      // [face_blocks]
+    uint32_t* rec_base;      // [face_blocks]: first overflow record of each face block
+    uint32_t* rec_pool;      // [kRecPools * kPoolStride]: pool counters; zero between draws
+    uint32_t rec_pools;      // overflow pools in use (1..kRecPools)
+    uint32_t pool_recs;      // records per pool: pool j is
+                             // [ovf_begin + j * pool_recs, ovf_begin + (j + 1) * pool_recs)
+    uint32_t ovf_begin;      // face_blocks * kRecsPerBlock
     uint32_t* refs;          // [refs_capacity]
     unsigned long long refs_capacity;
     unsigned long long* stats;  // [nframes * kNumStats]

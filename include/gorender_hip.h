@@ -163,7 +163,8 @@ typedef struct {
 /* Tunables (autotuned). Zero fields keep the current value. */
 typedef struct {
     int32_t tile_w, tile_h;       /* raster tile in pixels: 8, 16, 32 or 64 */
-    int64_t refs_capacity;        /* tile-list entries per batch (grown on demand) */
+    int64_t refs_capacity;        /* tile-list entries per batch, and triangle records
+                                     (at most 1792 per 256 faces); grown on demand */
     int32_t raster_waves;         /* waves per raster workgroup: 4 */
     int32_t tile_split;           /* tile-list entries per raster workgroup, 64..2^24: a
                                      longer list is split into parts that several
@@ -239,10 +240,11 @@ gr_status gr_draw_batch(gr_context *ctx, const gr_batch_frame *frames, int32_t n
  * lives in one fixed 32 MiB device array per GPU, held until the process ends;
  * host data reaches it through kernel arguments, and the draw calls no
  * allocation, free or memset. Limits: one frame of at most 16 objects, no edge
- * or vertex overlays (GR_ERR_UNSUPPORTED), and records + tile counters for the
- * frame plus 2^16 tile-list entries within 32 MiB (about 25k faces at 1080p):
- * larger frames -> GR_ERR_INVALID_ARGUMENT. Tile lists longer than what is
- * left of the array -> GR_ERR_CAPACITY from gr_synchronize. The array is
+ * or vertex overlays (GR_ERR_UNSUPPORTED), and the frame's tile counters, 320
+ * triangle records per 256 faces and 2^16 tile-list entries within 32 MiB
+ * (about 130k faces at 1080p): larger frames -> GR_ERR_INVALID_ARGUMENT. Tile
+ * lists or clipped triangles beyond what is left of the array ->
+ * GR_ERR_CAPACITY from gr_synchronize. The array is
  * shared by every call on the GPU and keeps the last call's contents: calls
  * that overlap on one GPU (two contexts or streams at once) race; calls on
  * one context's stream are ordered. The image equals gr_draw_device's. */

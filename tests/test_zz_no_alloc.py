@@ -2,7 +2,9 @@
 is a variant of (gr_draw_device), through the C ABI: the same colour and depth
 buffers bit for bit and the same counters, on two calls in a row, the second
 with a smaller frame and fewer faces (the fixed device array keeps the first
-call's contents). Sizes beyond the array are refused before any launch."""
+call's contents). Frames whose fixed parts exceed the array are refused
+before any launch; one that needs more triangle records or tile-list entries
+than the array holds is reported by gr_synchronize."""
 import numpy as np
 import pytest
 
@@ -65,9 +67,54 @@ def test_noalloc_equals_draw_device_twice(ctx):
         assert_same(scene, (got[0], got[1], got[2]), ro)
 
 
+def test_noalloc_frame_past_per_block_records(ctx):
+    # 50,000 faces: records reserved at 1792 per 256-face block would take
+    # 67 MB; 320 per block plus the overflow records fit the array.
+    scene = S.config2(nlat=125, nlon=200)
+    want = _draw(ctx, scene, noalloc=False)
+    got = _draw(ctx, scene, noalloc=True)
+    _same(scene, got, want)
+
+
+def test_noalloc_reports_capacity(ctx):
+    # Sixteen copies of a 480-face sphere filling a 4096 x 4096 frame in 8 x 8
+    # tiles: millions of tile-list entries, more than the array holds. The
+    # draw is refused at synchronize, as a capacity error of that array, and
+    # the context draws normally afterwards.
+    mesh = S.latlong_sphere(12, 20, S.SplitMix64(7))
+    fr = S.FrameData(4096, 4096)
+    obj = S.camera_object(fr, 0, (0, 0, 2.5), (0, 0, -1))
+    scene = S.SceneData(fr, [mesh], [], [obj] * 16, name="sixteen-spheres")
+    old = ctx.config()
+    ctx.set_config(tile_w=8, tile_h=8)
+    mh, th = ctx.upload_scene(scene)
+    rgba_b, depth_b = ctx.frame_buffers(fr.width, fr.height)
+    try:
+        ctx.draw_device_noalloc(scene, mh, rgba_b.ptr, depth_b.ptr)
+        with pytest.raises(_lib.GorenderError) as e:
+            ctx.synchronize()
+        assert e.value.status == _lib.GR_ERR_CAPACITY
+        assert "no-allocation" in str(e.value)
+        ctx.synchronize()
+    finally:
+        ctx.set_config(tile_w=old.tile_w, tile_h=old.tile_h)
+        rgba_b.free()
+        depth_b.free()
+        for h in mh:
+            ctx.destroy_mesh(h)
+        for h in th:
+            ctx.destroy_texture(h)
+    scene = C.sphere()
+    _same(scene, _draw(ctx, scene, noalloc=True), _draw(ctx, scene, noalloc=False))
+
+
 def test_noalloc_refuses_large_frames(ctx):
-    scene = S.config2()  # 200,000 faces: records alone exceed the array
+    # 16384 x 16384 pixels in 8 x 8 tiles: the tile counters, offsets and
+    # cursors alone (4M tiles) exceed the array; refused before any launch.
+    scene = C.sphere(w=16384, h=16384)
     fr = scene.frame
+    old = ctx.config()
+    ctx.set_config(tile_w=8, tile_h=8)
     mh, th = ctx.upload_scene(scene)
     rgba_b, depth_b = ctx.frame_buffers(fr.width, fr.height)
     try:
@@ -77,6 +124,7 @@ def test_noalloc_refuses_large_frames(ctx):
         assert "no-allocation" in str(e.value)
         ctx.synchronize()
     finally:
+        ctx.set_config(tile_w=old.tile_w, tile_h=old.tile_h)
         rgba_b.free()
         depth_b.free()
         for h in mh:
