@@ -663,6 +663,28 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                     csum += C;
                 }
                 rr.sf = (float)csum;
+                // A box of at most 2x2 pixels none of which passes the edge
+                // test (the raster's, rasterizer.go:147) yields no fragment: it
+                // stays out of the tile lists. Dense meshes far from the camera
+                // emit mostly such sub-pixel triangles.
+                if (mxx - mnx <= 1 && mxy - mny <= 1) {
+                    int32_t any = 0;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int32_t x = (int32_t)min(mnx + (k & 1), mxx);
+                        const int32_t y = (int32_t)min(mny + (k >> 1), mxy);
+                        const int32_t f0 = __mul24(rr.e[0], x) + __mul24(rr.e[1], y) + rr.e[2];
+                        const int32_t f1 = __mul24(rr.e[3], x) + __mul24(rr.e[4], y) + rr.e[5];
+                        const int32_t f2 = __mul24(rr.e[6], x) + __mul24(rr.e[7], y) + rr.e[8];
+                        any |= f0 & f1 & f2;
+                    }
+                    if (any >= 0) {
+                        rr.bx0 = 1;
+                        rr.bx1 = 0;
+                        rr.by0 = 1;
+                        rr.by1 = 0;
+                    }
+                }
             } else {
                 // Wrapping int64 arithmetic as the reference's int.
                 int64_t csum = 0;

@@ -205,3 +205,16 @@ def test_fog(ctx):
     assert (got != want_rgba).any(axis=-1).sum() > 1000
     rg.free()
     dp.free()
+
+
+def test_subpixel_triangles(ctx):
+    """A dense sphere far from the camera: most triangles fall between pixel
+    centres, and those whose box is at most 2x2 pixels with no covered pixel
+    stay out of the tile lists. The image and the counters still equal the
+    oracle's, and the lists hold fewer entries than there are triangles."""
+    import scene_cases as C
+    scene = C.sphere(w=160, h=96, eye=(0, 0, 14.0), nlat=100, nlon=160)
+    got = ctx.draw(scene)
+    assert_same(scene, got, refcpu.draw(scene))
+    st = got[2]
+    assert st["refs"] < st["triangles"], st
