@@ -4,8 +4,8 @@ Workload: the headline frame (displaced lat-long sphere of 200,000 triangles,
 16x16 RGBA8 texture, 1280x720, camera at r=1.2). Frame k rotates the object by
 0.01*k rad about Y, as the reference's demo loop does (main.go:229-233), so
 the frames of a launch are distinct work. A step is one batched launch of
---batch frames (64 by default, k = 0..63): the full path (vertex transform,
-clip, bin, raster, shade, colour + depth written to HBM) over 64 distinct
+--batch frames (256 by default, k = 0..255): the full path (vertex transform,
+clip, bin, raster, shade, colour + depth written to HBM) over 256 distinct
 frames, meshes and textures resident in HBM. --warmup steps run untimed first, then exactly --steps steps
 are timed between two device synchronisations.
 
@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=64,
                     help="timed steps; a step is one batched launch of --batch frames")
     ap.add_argument("--warmup", type=int, default=16, help="untimed steps before timing")
-    ap.add_argument("--batch", type=int, default=64, help="frames per batched launch")
+    ap.add_argument("--batch", type=int, default=256, help="frames per batched launch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -92,10 +92,14 @@ VALU_PEAK_GINSTR_S = 256 * 4 * 2.4 / 4
 
 def sq_counters(kernel, cfg, frames_per_launch):
     """SQ counters per launch of `kernel` from the committed rocprofv3 summary
-    of the headline batch, when it was taken with this tile and batch size."""
+    of the headline batch, when it was taken with this tile and batch size
+    (its first line names the frames per launch)."""
     path = profile_file("sq_counters_config2.txt")
-    if frames_per_launch != 64 or not os.path.exists(path):
+    if not os.path.exists(path):
         return {}
+    with open(path) as fh:
+        if f", {frames_per_launch} frames per launch" not in fh.readline():
+            return {}
     name = f"{kernel}<{cfg.tile_w}, {cfg.tile_h}>" if kernel.startswith("k_raster") else kernel
     tile_ok, out = False, {}
     for line in open(path):
