@@ -13,7 +13,8 @@ Forms timed: "graph" (each batch's launches captured in HIP graphs and
 replayed, two batches alternating on the context's two pipelines), "stream"
 (the same batches submitted with gr_draw_batch) and "plain" (one
 gr_draw_device call per frame, with the library's two pipelines and strictly
-one after another). The value is the faster batched form. Beside it: config 4
+one after another). The value is the faster batched form. Beside it: the
+graph form with the raster stage as a persistent grid, config 4
 (256 mixed frames: autotuned configuration, one graph launch for all 256
 against 256 plain launches, outputs compared on the device), configs 3 and 5,
 the serving loop at fixed request rates, per-kernel times, the HBM roofline of
@@ -512,6 +513,20 @@ def main():
 
         wall["graph"] = run_form(graph_step, Wm, K)
     stats = ctx.last_stats()[0]
+    persistent = None
+    if "graph" in modes and not args.profile:
+        # The raster stage as a persistent grid striding over the tiles
+        # (gr_config.raster_grid 2), same batches in the graph form: reported
+        # beside the default one workgroup per tile, never as `value`.
+        for g in graphs:
+            g.destroy()
+        ctx.set_config(raster_grid=2)
+        graphs = [ctx.batch_graph(b) for b in batches]
+        persistent = K * B / run_form(graph_step, Wm, K)
+        for g in graphs:
+            g.destroy()
+        graphs = []
+        ctx.set_config(raster_grid=1)
     if "plain" in modes:
         wall["plain"] = run_form(plain_step, min(Wm, 4), K)
         ctx.set_config(pipelines=1)
@@ -672,6 +687,9 @@ def main():
         "frames_per_s": {k: round(v, 2) for k, v in fps.items()},
         "batched_over_plain": (round(main_fps / max(fps["plain"], fps["plain_one_pipeline"]), 3)
                                if "plain" in fps else None),
+        "persistent_raster_graph": ({"frames_per_s": round(persistent, 2),
+                                     "vs_one_workgroup_per_tile": round(persistent / fps["graph"], 3)}
+                                    if persistent else None),
         "mtris_s": {k: round(faces * v / 1e6, 3) for k, v in fps.items()},
         "latency_ms": ({"device_p50": round(pct(lat_dev, 50), 4),
                         "device_p99": round(pct(lat_dev, 99), 4),

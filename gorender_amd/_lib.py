@@ -88,7 +88,7 @@ class gr_batch_frame(Structure):
 class gr_config(Structure):
     _fields_ = [("tile_w", c_int32), ("tile_h", c_int32), ("refs_capacity", c_int64),
                 ("raster_waves", c_int32), ("tile_split", c_int32), ("pipelines", c_int32),
-                ("reserved", c_int32)]
+                ("raster_grid", c_int32)]
 
 
 class gr_tune_result(Structure):

@@ -136,7 +136,7 @@ struct gr_context {
     int present_next = 0;  // slot of the next gr_present_draw
     int present_head = 0;  // oldest queued slot
 
-    gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 0};
+    gr_config cfg{32, 32, 1 << 22, 4, 512, 2, 1};
 
     bool timing = false;
     std::vector<TimedLaunch> timed;
@@ -405,6 +405,7 @@ struct Plan {
     int twl = 0, thl = 0;
     size_t slots = 0, okey_words = 0;
     bool decorate = false;  // a frame asks for tile borders or the crosshair
+    bool raster_persistent = false;  // k_raster as a persistent grid (gr_config.raster_grid 2)
     gr::DevFrame* frames = nullptr;  // [nf], caller storage
     gr::DevObject* dobj = nullptr;   // [nobj], caller storage
     int* xcd = nullptr;              // [8 * (nf + 1)], caller storage
@@ -424,6 +425,7 @@ gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_ob
     P = Plan{};
     P.nf = nf;
     P.twl = ilog2(TW);
+    P.raster_persistent = c->cfg.raster_grid == 2;
     P.thl = ilog2(TH);
     P.frames = frames;
     P.dobj = dobj;
@@ -617,6 +619,7 @@ gr::LaunchParams launch_params(const Plan& P, const Bufs& B) {
     p.ctrl = B.ctrl;
     p.xcd_face = (const int*)(bb + P.off_xcd);
     p.face_grid = P.face_grid;
+    p.raster_persistent = P.raster_persistent ? 1 : 0;
     return p;
 }
 
@@ -1366,6 +1369,9 @@ gr_status gr_set_config(gr_context* c, const gr_config* cfg) {
     if (cfg->raster_waves) n.raster_waves = cfg->raster_waves;
     if (cfg->tile_split) n.tile_split = cfg->tile_split;
     if (cfg->pipelines) n.pipelines = cfg->pipelines;
+    if (cfg->raster_grid) n.raster_grid = cfg->raster_grid;
+    if (n.raster_grid != 1 && n.raster_grid != 2)
+        return fail(GR_ERR_INVALID_ARGUMENT, "raster_grid must be 1 or 2");
     if (!valid_tile(n.tile_w) || !valid_tile(n.tile_h))
         return fail(GR_ERR_INVALID_ARGUMENT, "tile sides must be 8, 16, 32 or 64");
     if (n.raster_waves != 4) return fail(GR_ERR_INVALID_ARGUMENT, "raster_waves must be 4");

@@ -2040,27 +2040,21 @@ void k_raster_parts(const DevFrame* __restrict__ frames, int nframes,
     }
 }
 
-// One workgroup per screen tile: part 0 of its list (all of it when the tile is
-// not split), then the deferred shading.
+// Screen tile g: part 0 of its list (all of it when the tile is not split),
+// then the deferred shading. Every thread of the workgroup calls it.
 template <int TW, int TH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
-void k_raster(const DevFrame* __restrict__ frames, int nframes,
-              uint32_t* __restrict__ tile_counts, const uint32_t* __restrict__ tile_offsets,
-              const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
-              const DevTexture* __restrict__ textures, unsigned long long* __restrict__ stats,
-              unsigned long long* __restrict__ slabs, uint32_t part_refs) {
+__device__ __forceinline__ void raster_tile(
+    int g, const DevFrame* __restrict__ frames, int nframes, uint32_t* __restrict__ tile_counts,
+    const uint32_t* __restrict__ tile_offsets, const uint32_t* __restrict__ tile_slab,
+    const uint32_t* __restrict__ refs, const DevTexture* __restrict__ textures,
+    unsigned long long* __restrict__ stats, unsigned long long* __restrict__ slabs,
+    uint32_t part_refs, unsigned long long* key, uint4* lrec, uint32_t* lslot,
+    uint32_t (*qbuf)[3][128], unsigned long long (*red)[2], int lane, int wave) {
     constexpr int NPX = TW * TH;
-    __shared__ unsigned long long key[NPX];
-    __shared__ uint4 lrec[kRasterChunk * 5];  // the staged raster records
-    __shared__ uint32_t lslot[kRasterChunk];
-    __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
-    __shared__ unsigned long long red[4][2];  // per wave: covered fragments, shaded pixels
-    const int g = blockIdx.x;
     const int fi = last_le(nframes, g, [&](int i) { return frames[i].tile_begin; });
     const DevFrame& fr = frames[fi];
     const int t = g - fr.tile_begin;
     const int X0 = (t % fr.tiles_x) * TW, Y0 = (t / fr.tiles_x) * TH;
-    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
 
     for (int i = threadIdx.x; i < NPX; i += 256) key[i] = kBackgroundKey;
 
@@ -2097,6 +2091,52 @@ void k_raster(const DevFrame* __restrict__ frames, int nframes,
         const unsigned long long s = red[0][threadIdx.x] + red[1][threadIdx.x] +
                                      red[2][threadIdx.x] + red[3][threadIdx.x];
         if (s) atomicAdd(&stats[fi * kNumStats + (threadIdx.x ? kStatShaded : kStatCovered)], s);
+    }
+}
+
+// One workgroup per screen tile.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
+void k_raster(const DevFrame* __restrict__ frames, int nframes,
+              uint32_t* __restrict__ tile_counts, const uint32_t* __restrict__ tile_offsets,
+              const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
+              const DevTexture* __restrict__ textures, unsigned long long* __restrict__ stats,
+              unsigned long long* __restrict__ slabs, uint32_t part_refs) {
+    __shared__ unsigned long long key[TW * TH];
+    __shared__ uint4 lrec[kRasterChunk * 5];  // the staged raster records
+    __shared__ uint32_t lslot[kRasterChunk];
+    __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
+    __shared__ unsigned long long red[4][2];  // per wave: covered fragments, shaded pixels
+    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
+    raster_tile<TW, TH>((int)blockIdx.x, frames, nframes, tile_counts, tile_offsets, tile_slab,
+                        refs, textures, stats, slabs, part_refs, key, lrec, lslot, qbuf, red,
+                        lane, wave);
+}
+
+// Persistent form: a fixed grid (every CU at the raster's occupancy) in which
+// workgroup w draws tiles w, w + grid, w + 2 * grid, ...
+template <int TW, int TH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
+void k_raster_persistent(const DevFrame* __restrict__ frames, int nframes,
+                         uint32_t* __restrict__ tile_counts,
+                         const uint32_t* __restrict__ tile_offsets,
+                         const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
+                         const DevTexture* __restrict__ textures,
+                         unsigned long long* __restrict__ stats,
+                         unsigned long long* __restrict__ slabs, uint32_t part_refs,
+                         int total_tiles) {
+    __shared__ unsigned long long key[TW * TH];
+    __shared__ uint4 lrec[kRasterChunk * 5];
+    __shared__ uint32_t lslot[kRasterChunk];
+    __shared__ uint32_t qbuf[4][3][128];
+    __shared__ unsigned long long red[4][2];
+    const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
+    for (int g = (int)blockIdx.x; g < total_tiles; g += (int)gridDim.x) {
+        raster_tile<TW, TH>(g, frames, nframes, tile_counts, tile_offsets, tile_slab, refs,
+                            textures, stats, slabs, part_refs, key, lrec, lslot, qbuf, red, lane,
+                            wave);
+        // red and the LDS tile state are rewritten by the next tile.
+        __syncthreads();
     }
 }
 
@@ -2354,6 +2394,14 @@ static void raster_t(const LaunchParams& p, hipStream_t s, bool parts) {
             hipLaunchKernelGGL((k_raster_parts<TW, TH>), dim3(grid), dim3(256), 0, s, p.frames,
                                p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs,
                                p.stats, p.counters, p.items, p.max_items, p.slabs, p.part_refs);
+        return;
+    }
+    if (p.raster_persistent) {
+        const uint32_t grid =
+            std::min<uint32_t>((uint32_t)p.total_tiles, 256u * (uint32_t)raster_waves(TW * TH));
+        hipLaunchKernelGGL((k_raster_persistent<TW, TH>), dim3(grid), dim3(256), 0, s, p.frames,
+                           p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs,
+                           p.textures, p.stats, p.slabs, p.part_refs, p.total_tiles);
         return;
     }
     hipLaunchKernelGGL((k_raster<TW, TH>), dim3(p.total_tiles), dim3(256), 0, s, p.frames,

@@ -18,6 +18,7 @@ struct LaunchParams {
     const DevTexture* textures;
     int face_blocks;         // k_face / k_bin_fill blocks
     int total_tiles;         // raster workgroups
+    int raster_persistent;   // 1: k_raster as a fixed grid striding over the tiles
     int tw_log2, th_log2;    // raster tile size
     uint32_t* tile_counts;   // [total_tiles]; zero between draws (k_raster re-zeroes)
     uint32_t* tile_offsets;  // [total_tiles]

@@ -65,9 +65,9 @@ class Context:
 
     # ---- configuration -------------------------------------------------
     def set_config(self, tile_w: int = 0, tile_h: int = 0, refs_capacity: int = 0,
-                   tile_split: int = 0, pipelines: int = 0) -> None:
+                   tile_split: int = 0, pipelines: int = 0, raster_grid: int = 0) -> None:
         """Zero keeps the current value (gr_set_config)."""
-        c = _lib.gr_config(tile_w, tile_h, refs_capacity, 0, tile_split, pipelines, 0)
+        c = _lib.gr_config(tile_w, tile_h, refs_capacity, 0, tile_split, pipelines, raster_grid)
         _lib.check(self.lib.gr_set_config(self.handle, ctypes.byref(c)))
 
     def config(self) -> _lib.gr_config:

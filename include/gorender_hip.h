@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define GR_ABI_VERSION 4
+#define GR_ABI_VERSION 5
 
 typedef enum {
     GR_OK = 0,
@@ -174,7 +174,10 @@ typedef struct {
                                      draw's geometry stages overlap the previous draw's raster;
                                      frames are still written in submission order. A stream
                                      set with gr_set_stream runs everything on it (1). */
-    int32_t reserved;
+    int32_t raster_grid;          /* 1 (default): one raster workgroup per screen tile;
+                                     2: a persistent grid (every CU at the raster's
+                                     occupancy) whose workgroups stride over the tiles.
+                                     Same images; slower here, kept for comparison. */
 } gr_config;
 
 typedef struct gr_context gr_context;

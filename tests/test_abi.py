@@ -67,7 +67,7 @@ def test_struct_layout_matches_c(tmp_path):
 
 def test_abi_version_and_defaults():
     lib = _lib.load()
-    assert lib.gr_abi_version() == 4
+    assert lib.gr_abi_version() == 5
     f = _lib.gr_frame()
     lib.gr_frame_defaults(ctypes.byref(f), 800, 600)
     # NewRenderer / Draw defaults (renderer.go:12-17,118-136,448-449)

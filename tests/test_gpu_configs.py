@@ -141,3 +141,27 @@ def test_config5_split_tiles(ctx):
             check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
     finally:
         ctx.set_config(tile_split=before.tile_split)
+
+
+def test_persistent_raster_grid(ctx):
+    """gr_config.raster_grid = 2 (the raster stage as a persistent grid striding
+    over the tiles): the 256 config-4 frames in one batch and the config-5
+    serving batch with split tiles give the committed oracle digests."""
+    before = ctx.config()
+    ctx.set_config(raster_grid=2)
+    try:
+        assert ctx.config().raster_grid == 2
+        frames = config4_all()
+        handles = [ctx.upload_scene(s)[0] for s in frames]
+        for k, got in enumerate(draw_batch(ctx, frames, handles)):
+            check(f"config4-{k}", frames[k], got)
+        ctx.set_config(tile_split=64)
+        rs = config5_set()
+        th = [ctx.upload_texture(t) for t in rs.textures]
+        mh = [ctx.upload_mesh(m, th) for m in rs.meshes]
+        res = draw_batch(ctx, rs.requests, [mh] * len(rs.requests))
+        assert sum(r[2]["tiles_split"] for r in res) > 0
+        for k, got in enumerate(res):
+            check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
+    finally:
+        ctx.set_config(tile_split=before.tile_split, raster_grid=before.raster_grid)
