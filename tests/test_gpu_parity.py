@@ -218,3 +218,43 @@ def test_subpixel_triangles(ctx):
     assert_same(scene, got, refcpu.draw(scene))
     st = got[2]
     assert st["refs"] < st["triangles"], st
+
+
+def _straddling_fans(n=512, seed=9):
+    """n triangles given in clip space (identity matrices), each crossing the
+    x = -w, x = w and y = w planes: every face reaches the clipper and reserves
+    the largest fan, 7 records, so each 256-face block needs 1,792 records and
+    takes all but its first 320 from the overflow pools."""
+    rng = S.SplitMix64(S.SEED_BASE + 700 + seed)
+    # Inside the clip volume is w < 0, |x|, |y| <= -w and znear <= z <= zfar.
+    base = np.array([[-2.0, -0.2, 0.5, -1.0], [0.0, 2.0, 0.5, -1.0], [2.0, -0.1, 0.5, -1.0]])
+    jit = (rng.uniform(n * 12).reshape(n, 3, 4) - 0.5) * np.array([0.4, 0.4, 0.2, 0.0])
+    v4 = (base[None] + jit).reshape(3 * n, 4).astype(np.float32)
+    f = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0, 1.0]], np.float32), (3 * n, 1))
+    uvs = (rng.uniform(6 * n).reshape(n, 3, 2)).astype(np.float32)
+    m = S.MeshData(v4, f, nrm, f.copy(), uvs)
+    t = S.random_texture(rng, 8, 8)
+    S.with_texture(m, 0)
+    fr = S.FrameData(96, 72, flags=_lib.GR_DEFAULT_FLAGS & ~_lib.GR_BACKFACE_CULLING)
+    eye = np.eye(4, dtype=np.float32)
+    return S.SceneData(fr, [m], [t], [S.ObjectData(0, eye, eye)], name="straddling-fans")
+
+
+def test_overflow_records(ctx):
+    """Faces that all reach the clipper: most of their records come from the
+    overflow pools. Equal to the oracle at the default capacity, and with a
+    capacity so small that the pools run out (the batch reports it, the
+    capacity grows and the frame is drawn again)."""
+    scene = _straddling_fans()
+    want = refcpu.draw(scene)
+    assert want[2]["triangles"] > 0
+    assert_same(scene, ctx.draw(scene), want)
+    old = ctx.config()
+    try:
+        ctx.set_config(refs_capacity=1024)
+        got = ctx.draw(scene)
+        assert ctx.config().refs_capacity > 1024
+    finally:
+        ctx.set_config(refs_capacity=old.refs_capacity)
+    assert_same(scene, got, want)
