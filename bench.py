@@ -627,6 +627,17 @@ def main():
                       "achieved_ginstr_s": round(ach, 1), "peak_ginstr_s": VALU_PEAK_GINSTR_S,
                       "frac": round(ach / VALU_PEAK_GINSTR_S, 4),
                       "source": "SQ_INSTS_VALU, profiles/sq_counters_config2.txt"}
+        if sq.get("SQ_ACTIVE_INST_VALU2") is not None:
+            # Two VALU instructions can issue in one quad-cycle: the share that
+            # did, and the fraction of the launch a SIMD's VALU issue was busy
+            # counting such a pair once (1,024 SIMDs at 2.4 GHz, 4 cycles each).
+            pairs = sq["SQ_ACTIVE_INST_VALU2"]
+            busy = (sq["SQ_INSTS_VALU"] - pairs) * 4 / (1024 * 2.4e9 * avg_launch_s)
+            valu_issue["paired_frac"] = round(2 * pairs / sq["SQ_INSTS_VALU"], 4)
+            valu_issue["issue_busy_frac"] = round(busy, 4)
+        if sq.get("SQ_THREAD_CYCLES_VALU") is not None:
+            valu_issue["active_lanes_per_instr"] = round(
+                sq["SQ_THREAD_CYCLES_VALU"] / sq["SQ_INSTS_VALU"], 2)
     lds_conflict = (round(sq["SQ_LDS_BANK_CONFLICT"] / sq["SQ_ACTIVE_INST_LDS"], 4)
                     if sq.get("SQ_ACTIVE_INST_LDS") else None)
     occupancy = (round(sq["MEAN_WAVES_PER_SIMD"], 3) if sq.get("MEAN_WAVES_PER_SIMD") else None)
