@@ -1905,11 +1905,15 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             const RasterRec r = gload(&fr.rrec[ri]);
             const float4* rp = reinterpret_cast<const float4*>(&r);
             const ShadeRec s = gload(&fr.srec[ri]);
-            float f12f, f20f;
-            if (!r.wide) {
-                f12f = (float)(__mul24(r.e[3], x) + __mul24(r.e[4], y) + r.e[5]);
-                f20f = (float)(__mul24(r.e[6], x) + __mul24(r.e[7], y) + r.e[8]);
-            } else {
+            // The int32 edge values are formed before the wide test, so that
+            // the whole record is read in one round trip; a wide record
+            // replaces them (its int32 values are never used, and the sums
+            // wrap as unsigned).
+            float f12f = (float)(int32_t)((uint32_t)__mul24(r.e[3], x) +
+                                          (uint32_t)__mul24(r.e[4], y) + (uint32_t)r.e[5]);
+            float f20f = (float)(int32_t)((uint32_t)__mul24(r.e[6], x) +
+                                          (uint32_t)__mul24(r.e[7], y) + (uint32_t)r.e[8]);
+            if (r.wide) {
                 int64_t f[3];
                 wide_edges(&fr.wrec[ri], x, y, f);
                 f12f = (float)f[1];
