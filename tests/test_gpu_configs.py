@@ -165,3 +165,17 @@ def test_persistent_raster_grid(ctx):
             check(f"config5-{k}", rs.requests[k], got, S.subscene(rs.requests[k]))
     finally:
         ctx.set_config(tile_split=before.tile_split, raster_grid=before.raster_grid)
+
+
+def test_config_validation(ctx):
+    """gr_set_config refuses out-of-range fields and keeps the configuration."""
+    before = ctx.config()
+    for kw in ({"raster_grid": 3}, {"tile_w": 24}, {"tile_split": 32}, {"pipelines": 3}):
+        with pytest.raises(_lib.GorenderError) as e:
+            ctx.set_config(**kw)
+        assert e.value.status == _lib.GR_ERR_INVALID_ARGUMENT, kw
+        now = ctx.config()
+        assert (now.tile_w, now.tile_h, now.tile_split, now.pipelines, now.raster_grid) == (
+            before.tile_w, before.tile_h, before.tile_split, before.pipelines,
+            before.raster_grid), kw
+    assert before.raster_grid == 1
