@@ -933,7 +933,7 @@ __device__ __forceinline__ void wave_sync() {
 // at barriers holding registers, and every resident wave works.
 constexpr int kFacesPerLane = kFacesPerWg / 64;
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_face(const DevObject* __restrict__ objs, int nobj,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_face(const DevObject* __restrict__ objs, int nobj,
                                               const DevFrame* __restrict__ frames,
                                               uint32_t* __restrict__ tile_counts,
                                               uint32_t* __restrict__ wg_count,
