@@ -2098,23 +2098,34 @@ __device__ __forceinline__ void raster_tile(
     }
 }
 
-// One workgroup per screen tile.
+// XCD-aware tile order. Workgroup h runs on XCD h mod 8; each XCD takes runs
+// of kTileRun consecutive tiles (runs x, x + 8, x + 16, ... for XCD x), so
+// neighbouring tiles, which share triangle records, read them through one L2.
+constexpr int kTileRun = 64;
+__device__ __forceinline__ int xcd_tile(int h) {
+    const int x = h & 7, j = h >> 3;
+    return ((j / kTileRun) * 8 + x) * kTileRun + j % kTileRun;
+}
+
+// One workgroup per screen tile (grid: total_tiles rounded up to whole runs
+// of all XCDs; workgroups past the last tile return at once).
 template <int TW, int TH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(raster_waves(TW * TH))))
 void k_raster(const DevFrame* __restrict__ frames, int nframes,
               uint32_t* __restrict__ tile_counts, const uint32_t* __restrict__ tile_offsets,
               const uint32_t* __restrict__ tile_slab, const uint32_t* __restrict__ refs,
               const DevTexture* __restrict__ textures, unsigned long long* __restrict__ stats,
-              unsigned long long* __restrict__ slabs, uint32_t part_refs) {
+              unsigned long long* __restrict__ slabs, uint32_t part_refs, int total_tiles) {
+    const int g = xcd_tile((int)blockIdx.x);
+    if (g >= total_tiles) return;
     __shared__ unsigned long long key[TW * TH];
     __shared__ uint4 lrec[kRasterChunk * 5];  // the staged raster records
     __shared__ uint32_t lslot[kRasterChunk];
     __shared__ uint32_t qbuf[4][3][128];  // per-wave ring: pixel, f12, f20
     __shared__ unsigned long long red[4][2];  // per wave: covered fragments, shaded pixels
     const int lane = threadIdx.x & 63, wave = sgpr(threadIdx.x >> 6);
-    raster_tile<TW, TH>((int)blockIdx.x, frames, nframes, tile_counts, tile_offsets, tile_slab,
-                        refs, textures, stats, slabs, part_refs, key, lrec, lslot, qbuf, red,
-                        lane, wave);
+    raster_tile<TW, TH>(g, frames, nframes, tile_counts, tile_offsets, tile_slab, refs, textures,
+                        stats, slabs, part_refs, key, lrec, lslot, qbuf, red, lane, wave);
 }
 
 // Persistent form: a fixed grid (every CU at the raster's occupancy) in which
@@ -2408,9 +2419,11 @@ static void raster_t(const LaunchParams& p, hipStream_t s, bool parts) {
                            p.textures, p.stats, p.slabs, p.part_refs, p.total_tiles);
         return;
     }
-    hipLaunchKernelGGL((k_raster<TW, TH>), dim3(p.total_tiles), dim3(256), 0, s, p.frames,
-                       p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs, p.textures,
-                       p.stats, p.slabs, p.part_refs);
+    constexpr int kRound = 8 * kTileRun;
+    const uint32_t grid = (uint32_t)((p.total_tiles + kRound - 1) / kRound * kRound);
+    hipLaunchKernelGGL((k_raster<TW, TH>), dim3(grid), dim3(256), 0, s, p.frames, p.nframes,
+                       p.tile_counts, p.tile_offsets, p.tile_slab, p.refs, p.textures, p.stats,
+                       p.slabs, p.part_refs, p.total_tiles);
 }
 
 static bool raster_dispatch(const LaunchParams& p, hipStream_t s, bool parts) {
