@@ -403,7 +403,7 @@ gr_status check_frame(gr_context* c, const gr_frame& fr, const gr_object* objs, 
 struct Plan {
     int nf = 0, nobj = 0, fblocks = 0, tiles = 0;
     int twl = 0, thl = 0;
-    size_t slots = 0, okey_words = 0;
+    size_t okey_words = 0;
     bool decorate = false;  // a frame asks for tile borders or the crosshair
     bool raster_persistent = false;  // k_raster as a persistent grid (gr_config.raster_grid 2)
     gr::DevFrame* frames = nullptr;  // [nf], caller storage
@@ -500,7 +500,6 @@ gr_status make_plan(gr_context* c, const gr_batch_frame* bf, int nf, const gr_ob
         d.srec = nullptr;
         d.wrec = nullptr;
         d.rbase = nullptr;
-        P.slots += (size_t)d.face_blocks * gr::kSlotsPerWg;
     }
     P.nobj = nobj;
     P.off_objs = align256((size_t)nf * sizeof(gr::DevFrame));
