@@ -810,36 +810,6 @@ __device__ __forceinline__ TileRect rec_tiles(const RasterRec& r, int twl, int t
     return t;
 }
 
-// Union of the tile ranges of the records of offsets [0, cnt) of face block gb
-// (overflow records from ovf), into lrect (LDS, 4 ints) by LDS atomics. Ends
-// with a barrier.
-__device__ void block_tile_rect(const RasterRec* rr, uint32_t gb, uint32_t ovf, uint32_t cnt,
-                                int twl, int thl, int* lrect) {
-    if (threadIdx.x == 0) {
-        lrect[0] = 0x7FFFFFFF;
-        lrect[1] = -1;
-        lrect[2] = 0x7FFFFFFF;
-        lrect[3] = -1;
-    }
-    __syncthreads();
-    int a = 0x7FFFFFFF, b = -1, c = 0x7FFFFFFF, d = -1;
-    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-        TileRect t = rec_tiles(rr[block_rec(gb, ovf, i)], twl, thl);
-        if (t.tx0 > t.tx1) continue;
-        a = min(a, t.tx0);
-        b = max(b, t.tx1);
-        c = min(c, t.ty0);
-        d = max(d, t.ty1);
-    }
-    if (b >= 0) {
-        atomicMin(&lrect[0], a);
-        atomicMax(&lrect[1], b);
-        atomicMin(&lrect[2], c);
-        atomicMax(&lrect[3], d);
-    }
-    __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // k_face
 // ---------------------------------------------------------------------------
@@ -850,9 +820,10 @@ __device__ void block_tile_rect(const RasterRec* rr, uint32_t gb, uint32_t ovf, 
 // path, whose numerators are nonzero int32 edge values and barycentrics with
 // |alpha|, |beta| >= 2^-41 and gamma = +0 or |gamma| >= 2^-64 (left 0, i.e.
 // IEEE division, for int64 triangles and divisors outside rcp_range).
-__device__ __forceinline__ void finalize_record(RasterRec* r, ShadeRec* sr) {
+__device__ __forceinline__ TileRect finalize_record(RasterRec* r, ShadeRec* sr, int twl, int thl) {
     const float4 d = *reinterpret_cast<const float4*>(r);
-    if (r->bx0 > r->bx1) return;  // not rasterised; no shading inputs were written
+    const TileRect t = rec_tiles(*r, twl, thl);
+    if (r->bx0 > r->bx1) return t;  // not rasterised; no shading inputs were written
     sr->uz[0] = sr->uz[0] / d.y;
     sr->vz[0] = sr->vz[0] / d.y;
     sr->uz[1] = sr->uz[1] / d.z;
@@ -862,6 +833,7 @@ __device__ __forceinline__ void finalize_record(RasterRec* r, ShadeRec* sr) {
     if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w))
         reinterpret_cast<float4*>(r)[1] =
             make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
+    return t;
 }
 
 // Exclusive block scan of small counts (256 threads = 4 waves).
@@ -1446,7 +1418,7 @@ __global__ __launch_bounds__(1024) void k_bin_scan(const DevFrame* __restrict__ 
 // k_bin_fill
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __restrict__ objs, int nobj,
+__global__ __launch_bounds__(kFillThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_fill(const DevObject* __restrict__ objs, int nobj,
                                                   const DevFrame* __restrict__ frames,
                                                   const uint32_t* __restrict__ wg_count,
                                                   const uint32_t* __restrict__ frame_overflow,
@@ -1466,37 +1438,80 @@ __global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __re
     const uint32_t blk = (uint32_t)(o.seq_block + (b - o.face_block));  // in the frame
     const uint32_t slot0 = blk * kSlotsPerWg;
     const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? fr.rbase[blk] : 0u;
-    const RasterRec* rr = fr.rrec;
-    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-        const uint32_t ri = block_rec(b, ovf, i);
-        finalize_record(&fr.rrec[ri], &fr.srec[ri]);
+    // Tile ranges of the block's first kFillKeep * kFillThreads records stay in
+    // registers from the finalising pass to the placing pass (the records are
+    // read once); later records, if any, are read again in each pass.
+    constexpr int kFillKeep = 4;
+    TileRect tr[kFillKeep];
+    if (threadIdx.x == 0) {
+        lrect[0] = 0x7FFFFFFF;
+        lrect[1] = -1;
+        lrect[2] = 0x7FFFFFFF;
+        lrect[3] = -1;
     }
-    // The barrier in block_tile_rect orders these writes before the reads below.
-    block_tile_rect(rr, b, ovf, cnt, twl, thl, lrect);
+    int ax = 0x7FFFFFFF, bx = -1, ay = 0x7FFFFFFF, by = -1;
+    auto fold = [&](const TileRect& t) {
+        if (t.tx0 > t.tx1) return;
+        ax = min(ax, t.tx0);
+        bx = max(bx, t.tx1);
+        ay = min(ay, t.ty0);
+        by = max(by, t.ty1);
+    };
+#pragma unroll
+    for (int k = 0; k < kFillKeep; k++) {
+        const uint32_t i = threadIdx.x + (uint32_t)(k * kFillThreads);
+        tr[k] = TileRect{1, 0, 1, 0};
+        if (i < cnt) {
+            const uint32_t ri = block_rec(b, ovf, i);
+            tr[k] = finalize_record(&fr.rrec[ri], &fr.srec[ri], twl, thl);
+            fold(tr[k]);
+        }
+    }
+    for (uint32_t i = threadIdx.x + kFillKeep * kFillThreads; i < cnt; i += kFillThreads) {
+        const uint32_t ri = block_rec(b, ovf, i);
+        fold(finalize_record(&fr.rrec[ri], &fr.srec[ri], twl, thl));
+    }
+    // Every tile range of record i, from registers or read again.
+    auto each_range = [&](auto&& fn) {
+#pragma unroll
+        for (int k = 0; k < kFillKeep; k++) {
+            const uint32_t i = threadIdx.x + (uint32_t)(k * kFillThreads);
+            if (i < cnt) fn(tr[k], i);
+        }
+        for (uint32_t i = threadIdx.x + kFillKeep * kFillThreads; i < cnt; i += kFillThreads)
+            fn(rec_tiles(fr.rrec[block_rec(b, ovf, i)], twl, thl), i);
+    };
+    // The union of the ranges, by LDS atomics.
+    __syncthreads();
+    if (bx >= 0) {
+        atomicMin(&lrect[0], ax);
+        atomicMax(&lrect[1], bx);
+        atomicMin(&lrect[2], ay);
+        atomicMax(&lrect[3], by);
+    }
+    __syncthreads();
     const int rx0 = lrect[0], rx1 = lrect[1], ry0 = lrect[2], ry1 = lrect[3];
     if (rx1 < 0) return;
     const int rw = rx1 - rx0 + 1, rh = ry1 - ry0 + 1;
     if (rw * rh > kLdsTiles) {
         // Very large rectangle: one global atomic per reference.
-        for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-            TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
+        each_range([&](const TileRect& t, uint32_t i) {
             for (int ty = t.ty0; ty <= t.ty1; ty++)
                 for (int tx = t.tx0; tx <= t.tx1; tx++) {
                     const uint32_t pos =
                         atomicAdd(&tile_cursor[fr.tile_begin + ty * fr.tiles_x + tx], 1u);
                     refs[pos] = slot0 + i;
                 }
-        }
+        });
         return;
     }
     for (int i = threadIdx.x; i < rw * rh; i += kFillThreads) lcnt[i] = 0u;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-        TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
+    each_range([&](const TileRect& t, uint32_t) {
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++)
                 atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
-    }
+    });
     __syncthreads();
     // One global reservation per touched tile; lcnt becomes the write cursor.
     for (int i = threadIdx.x; i < rw * rh; i += kFillThreads) {
@@ -1507,14 +1522,13 @@ __global__ __launch_bounds__(kFillThreads) void k_bin_fill(const DevObject* __re
                                 v);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < cnt; i += kFillThreads) {
-        TileRect t = rec_tiles(rr[block_rec(b, ovf, i)], twl, thl);
+    each_range([&](const TileRect& t, uint32_t i) {
         for (int ty = t.ty0; ty <= t.ty1; ty++)
             for (int tx = t.tx0; tx <= t.tx1; tx++) {
                 const uint32_t pos = atomicAdd(&lcnt[(ty - ry0) * rw + (tx - rx0)], 1u);
                 refs[pos] = slot0 + i;
             }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
