@@ -2416,9 +2416,12 @@ template <int TW, int TH>
 static void raster_t(const LaunchParams& p, hipStream_t s, bool parts) {
     if (parts) {
         // Workgroups of the parts kernel: enough to fill every CU at the
-        // raster's occupancy; with no work items they return at once.
-        const uint32_t grid =
-            std::min<uint32_t>(p.max_items, 256u * (uint32_t)raster_waves(TW * TH));
+        // raster's occupancy, but no more than the batch has tiles (a single
+        // frame rarely splits many); with no work items they return at once,
+        // and any number of them drains the queue.
+        const uint32_t grid = std::min<uint32_t>(
+            std::min<uint32_t>(p.max_items, 256u * (uint32_t)raster_waves(TW * TH)),
+            std::max<uint32_t>(256u, (uint32_t)p.total_tiles));
         if (grid)
             hipLaunchKernelGGL((k_raster_parts<TW, TH>), dim3(grid), dim3(256), 0, s, p.frames,
                                p.nframes, p.tile_counts, p.tile_offsets, p.tile_slab, p.refs,
