@@ -135,14 +135,18 @@ struct alignas(16) RasterRec {
 };
 static_assert(sizeof(RasterRec) == 80, "RasterRec is five 16-byte pieces");
 
-// Per-triangle shading inputs, 48 bytes.
+// Per-triangle shading inputs, 48 bytes: with pieces 3 and 4 of the RasterRec
+// (the edge coefficients of edges 12 and 20), all the deferred shading of a
+// pixel reads, unless its zRec is zero.
 struct alignas(16) ShadeRec {
     float uz[3];  // u_i / z_i (rasterizer.go:132-137)
     float vz[3];
     float in[3];  // intensities
     int32_t tex;  // texture slot or -1
-    int32_t pad[2];
+    float sf;     // the RasterRec's sf
+    uint32_t rsf; // its rsf (0: IEEE division), or kShadeWide for a WideRec triangle
 };
+constexpr uint32_t kShadeWide = 0x7FC0DEADu;  // a NaN: never a reciprocal
 
 // Screen positions of a post-clip triangle for the debug overlays, 32 bytes.
 struct alignas(16) OverlayRec {

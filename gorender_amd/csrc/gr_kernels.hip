@@ -725,7 +725,8 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
             sr.in[1] = b.i;
             sr.in[2] = c.i;
             sr.tex = tex;
-            sr.pad[0] = sr.pad[1] = 0;
+            sr.sf = rr.sf;
+            sr.rsf = rr.wide ? kShadeWide : 0u;  // finalize_record sets a reciprocal
             fr.srec[ri] = sr;
         }
     }
@@ -830,9 +831,11 @@ __device__ __forceinline__ TileRect finalize_record(RasterRec* r, ShadeRec* sr, 
     sr->vz[1] = sr->vz[1] / d.z;
     sr->uz[2] = sr->uz[2] / d.w;
     sr->vz[2] = sr->vz[2] / d.w;
-    if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w))
-        reinterpret_cast<float4*>(r)[1] =
-            make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
+    if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w)) {
+        const float4 rc = make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
+        reinterpret_cast<float4*>(r)[1] = rc;
+        sr->rsf = __float_as_uint(rc.x);  // the shading pass reads it from here
+    }
     return t;
 }
 
@@ -1916,18 +1919,19 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             depth = -1.0f;
         } else {
             const uint32_t ri = rec_index(fr, seq - 1u);
-            const RasterRec r = gload(&fr.rrec[ri]);
-            const float4* rp = reinterpret_cast<const float4*>(&r);
+            // The shading record and pieces 3 and 4 of the raster record (B01,
+            // C01, A12, B12 | C12, A20, B20, C20), in one round trip.
+            const uint4* rq = reinterpret_cast<const uint4*>(&fr.rrec[ri]);
             const ShadeRec s = gload(&fr.srec[ri]);
-            // The int32 edge values are formed before the wide test, so that
-            // the whole record is read in one round trip; a wide record
-            // replaces them (its int32 values are never used, and the sums
-            // wrap as unsigned).
-            float f12f = (float)(int32_t)((uint32_t)__mul24(r.e[3], x) +
-                                          (uint32_t)__mul24(r.e[4], y) + (uint32_t)r.e[5]);
-            float f20f = (float)(int32_t)((uint32_t)__mul24(r.e[6], x) +
-                                          (uint32_t)__mul24(r.e[7], y) + (uint32_t)r.e[8]);
-            if (r.wide) {
+            const uint4 e3 = gload(rq + 3), e4 = gload(rq + 4);
+            // The int32 edge values are formed before the wide test; a wide
+            // record replaces them (its int32 values are never used, and the
+            // sums wrap as unsigned).
+            float f12f = (float)(int32_t)((uint32_t)__mul24((int)e3.z, x) +
+                                          (uint32_t)__mul24((int)e3.w, y) + e4.x);
+            float f20f = (float)(int32_t)((uint32_t)__mul24((int)e4.y, x) +
+                                          (uint32_t)__mul24((int)e4.z, y) + e4.w);
+            if (s.rsf == kShadeWide) {
                 int64_t f[3];
                 wide_edges(&fr.wrec[ri], x, y, f);
                 f12f = (float)f[1];
@@ -1939,18 +1943,22 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             const float zkey = __uint_as_float((ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord);
             Bary b;
             if (zkey != 0.0f) {
-                const float4 d = rp[0], rc = rp[1];
-                if (rc.x != 0.f) {
-                    b.alpha = div_rcp(f12f, d.x, rc.x);
-                    b.beta = div_rcp(f20f, d.x, rc.x);
+                // rsf is 0 or kShadeWide (IEEE division) unless the raster
+                // record holds reciprocals.
+                const float rsf = __uint_as_float(s.rsf);
+                if (s.rsf != 0u && s.rsf != kShadeWide) {
+                    b.alpha = div_rcp(f12f, s.sf, rsf);
+                    b.beta = div_rcp(f20f, s.sf, rsf);
                 } else {
-                    b.alpha = f12f / d.x;
-                    b.beta = f20f / d.x;
+                    b.alpha = f12f / s.sf;
+                    b.beta = f20f / s.sf;
                 }
                 b.gamma = 1.0f - b.alpha - b.beta;
                 b.zrec = zkey;
             } else {
-                b = bary(f12f, f20f, rp[0], rp[1]);
+                const float4 d = gload(reinterpret_cast<const float4*>(rq)),
+                             rc = gload(reinterpret_cast<const float4*>(rq) + 1);
+                b = bary(f12f, f20f, d, rc);
             }
             // rasterizer.go:158-170
             float u = (b.alpha * s.uz[0] + b.beta * s.uz[1] + b.gamma * s.uz[2]) / b.zrec;
