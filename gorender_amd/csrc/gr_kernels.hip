@@ -766,11 +766,23 @@ __device__ __forceinline__ uint32_t block_rec(uint32_t gb, uint32_t ovf, uint32_
 
 // Record of slot `slot` of a frame (the block's overflow start is read only
 // past its own range).
+// Scalar load and store through a global-address-space pointer: the pointers
+// of DevFrame are generic, and a generic (flat) access also counts against
+// the LDS wait counter, so a later LDS wait would wait for it too.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
 __device__ __forceinline__ uint32_t rec_index(const DevFrame& fr, uint32_t slot) {
     const uint32_t blk = slot / (uint32_t)kSlotsPerWg, off = slot % (uint32_t)kSlotsPerWg;
     return off < (uint32_t)kRecsPerBlock
                ? ((uint32_t)fr.face_block_begin + blk) * (uint32_t)kRecsPerBlock + off
-               : fr.rbase[blk] + (off - (uint32_t)kRecsPerBlock);
+               : gld(&fr.rbase[blk]) + (off - (uint32_t)kRecsPerBlock);
 }
 
 // General faces clipped per round by one wave; each fills kMaxFan lanes.
@@ -1610,14 +1622,14 @@ __device__ __forceinline__ uint32_t sample(const DevTexture& t, float u, float v
     case 2: {
         const int32_t x = go_int_lo((1 - u) * t.scale * t.wf) & (t.w - 1);
         const int32_t y = go_int_lo(v * t.scale * t.hf) & (t.h - 1);
-        return t.texels[y * t.w + x];
+        return gld(&t.texels[y * t.w + x]);
     }
     case 1: {
         const int32_t x = go_int_rem((1 - u) * t.scale * t.wf, t.w);
         const int32_t y = go_int_rem(v * t.scale * t.hf, t.h);
         int32_t idx = y * t.w + x;
         if (idx < 0) idx = 0;
-        return t.texels[idx];
+        return gld(&t.texels[idx]);
     }
     default:
         return 0xFFFF00FFu;  // magenta {255,0,255,255}
@@ -1964,22 +1976,42 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             float u = (b.alpha * s.uz[0] + b.beta * s.uz[1] + b.gamma * s.uz[2]) / b.zrec;
             float v = (b.alpha * s.vz[0] + b.beta * s.vz[1] + b.gamma * s.vz[2]) / b.zrec;
             float inten = b.alpha * s.in[0] + b.beta * s.in[1] + b.gamma * s.in[2];
-            uint32_t c = s.tex < 0 ? fr.face_color : sample(textures[s.tex], u, v);
+            uint32_t c;
+            const int tex0 = sgpr(s.tex);
+            if (__ballot(s.tex != tex0) == 0ull) {
+                // One texture (or none) for the wave's pixels: its descriptor
+                // is read once, with scalar loads.
+                if (tex0 < 0) {
+                    c = fr.face_color;
+                } else {
+                    static_assert(sizeof(DevTexture) == 40, "ten words");
+                    typedef const __attribute__((address_space(4))) uint32_t* cword;
+                    const cword src = (cword)(textures + tex0);
+                    uint32_t w[10];
+#pragma unroll
+                    for (int k = 0; k < 10; k++) w[k] = src[k];
+                    DevTexture t;
+                    __builtin_memcpy(&t, w, sizeof(DevTexture));
+                    c = sample(t, u, v);
+                }
+            } else {
+                c = s.tex < 0 ? fr.face_color : sample(textures[s.tex], u, v);
+            }
             color = color_intensity(c, inten);
             depth = b.zrec;
         }
         const size_t idx = (size_t)y * fr.W + x;
         if (fr.okey) {
             // Overlay keys are zero between draws: each pixel's is read once.
-            const uint32_t ok = fr.okey[idx];
+            const uint32_t ok = gld(&fr.okey[idx]);
             if (ok) {
-                fr.okey[idx] = 0u;
+                gst(&fr.okey[idx], 0u);
                 if ((ok >> 1) >= seq)
                     color = (ok & 1u) ? kVertexColor : ((fr.flags & kShowFaces) ? kEdgeColor : kWhite);
             }
         }
-        ((uint32_t*)fr.rgba)[idx] = color;
-        fr.depth[idx] = depth;
+        gst(&((uint32_t*)fr.rgba)[idx], color);
+        gst(&fr.depth[idx], depth);
     }
     return shaded;
 }
