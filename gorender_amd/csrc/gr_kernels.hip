@@ -73,6 +73,46 @@ __device__ __forceinline__ int64_t wmul(int64_t a, int64_t b) {
     return (int64_t)((uint64_t)a * (uint64_t)b);
 }
 
+// Scalar load and store through a global-address-space pointer: the pointers
+// of DevFrame are generic, and a generic (flat) access also counts against
+// the LDS wait counter, so a later LDS wait would wait for it too.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+// Load of a 16-byte-aligned record through a global-address-space pointer
+// (global_load rather than flat, so LDS traffic is not waited on with it).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "16-byte records only");
+    typedef const __attribute__((address_space(1))) u32x4* gptr;
+    const gptr src = (gptr)p;
+    u32x4 tmp[sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); i++) tmp[i] = src[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
+
+// Store of a 16-byte-aligned record through a global-address-space pointer.
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, const T& v) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "16-byte records only");
+    typedef __attribute__((ext_vector_type(4))) uint32_t v4;
+    typedef __attribute__((address_space(1))) v4* gptr;
+    v4 tmp[sizeof(T) / 16];
+    __builtin_memcpy(tmp, &v, sizeof(T));
+    const gptr dst = (gptr)p;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); i++) dst[i] = tmp[i];
+}
+
 // ---------------------------------------------------------------------------
 // Exact float32 division from a correctly rounded reciprocal
 // ---------------------------------------------------------------------------
@@ -609,7 +649,7 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
         }
         o.binned = binned ? 1u : 0u;
         o.pad = 0;
-        fr.orec[ri] = o;
+        gstore(&fr.orec[ri], o);
     }
     if (binned) {
         tris = 1;
@@ -711,7 +751,7 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                 }
                 wr.adj = adj;
                 wr.pad[0] = wr.pad[1] = wr.pad[2] = 0;
-                fr.wrec[ri] = wr;
+                gstore(&fr.wrec[ri], wr);
             }
             ShadeRec sr;
             // u, v as clipped; finalize_record divides them by z (rasterizer.go:132-137).
@@ -727,10 +767,10 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
             sr.tex = tex;
             sr.sf = rr.sf;
             sr.rsf = rr.wide ? kShadeWide : 0u;  // finalize_record sets a reciprocal
-            fr.srec[ri] = sr;
+            gstore(&fr.srec[ri], sr);
         }
     }
-    fr.rrec[ri] = rr;
+    gstore(&fr.rrec[ri], rr);
     if (rr.bx0 <= rr.bx1)
         count_tiles(fr, tally, (int)rr.bx0 >> twl, (int)rr.bx1 >> twl, (int)rr.by0 >> thl,
                     (int)rr.by1 >> thl);
@@ -750,10 +790,10 @@ __device__ __forceinline__ void empty_record(const DevFrame& fr, uint32_t ri) {
     rr.wide = 0;
 #pragma unroll
     for (int e = 0; e < 9; e++) rr.e[e] = 0;
-    fr.rrec[ri] = rr;
+    gstore(&fr.rrec[ri], rr);
     if (fr.orec) {
         OverlayRec o = {};
-        fr.orec[ri] = o;
+        gstore(&fr.orec[ri], o);
     }
 }
 
@@ -766,18 +806,6 @@ __device__ __forceinline__ uint32_t block_rec(uint32_t gb, uint32_t ovf, uint32_
 
 // Record of slot `slot` of a frame (the block's overflow start is read only
 // past its own range).
-// Scalar load and store through a global-address-space pointer: the pointers
-// of DevFrame are generic, and a generic (flat) access also counts against
-// the LDS wait counter, so a later LDS wait would wait for it too.
-template <typename T>
-__device__ __forceinline__ T gld(const T* p) {
-    return *(const __attribute__((address_space(1))) T*)p;
-}
-template <typename T>
-__device__ __forceinline__ void gst(T* p, T v) {
-    *(__attribute__((address_space(1))) T*)p = v;
-}
-
 __device__ __forceinline__ uint32_t rec_index(const DevFrame& fr, uint32_t slot) {
     const uint32_t blk = slot / (uint32_t)kSlotsPerWg, off = slot % (uint32_t)kSlotsPerWg;
     return off < (uint32_t)kRecsPerBlock
@@ -807,20 +835,28 @@ struct TileRect {
 };
 
 // Tile range of a raster record (empty: tx0 > tx1).
-__device__ __forceinline__ TileRect rec_tiles(const RasterRec& r, int twl, int thl) {
+// Tile range of a record's box from its piece 2 (bx0 | bx1 << 16, by0 | by1 << 16,
+// wide, A01); an empty box gives an empty range.
+__device__ __forceinline__ TileRect box_tiles(const uint4& p2, int twl, int thl) {
+    const int bx0 = (int)(int16_t)(p2.x & 0xFFFFu), bx1 = (int)(int16_t)(p2.x >> 16);
+    const int by0 = (int)(int16_t)(p2.y & 0xFFFFu), by1 = (int)(int16_t)(p2.y >> 16);
     TileRect t;
-    if (r.bx0 > r.bx1) {
+    if (bx0 > bx1) {
         t.tx0 = 1;
         t.tx1 = 0;
         t.ty0 = 1;
         t.ty1 = 0;
     } else {
-        t.tx0 = (int)r.bx0 >> twl;
-        t.tx1 = (int)r.bx1 >> twl;
-        t.ty0 = (int)r.by0 >> thl;
-        t.ty1 = (int)r.by1 >> thl;
+        t.tx0 = bx0 >> twl;
+        t.tx1 = bx1 >> twl;
+        t.ty0 = by0 >> thl;
+        t.ty1 = by1 >> thl;
     }
     return t;
+}
+
+__device__ __forceinline__ TileRect rec_tiles(const RasterRec* r, int twl, int thl) {
+    return box_tiles(gload(reinterpret_cast<const uint4*>(r) + 2), twl, thl);
 }
 
 // ---------------------------------------------------------------------------
@@ -834,19 +870,27 @@ __device__ __forceinline__ TileRect rec_tiles(const RasterRec& r, int twl, int t
 // |alpha|, |beta| >= 2^-41 and gamma = +0 or |gamma| >= 2^-64 (left 0, i.e.
 // IEEE division, for int64 triangles and divisors outside rcp_range).
 __device__ __forceinline__ TileRect finalize_record(RasterRec* r, ShadeRec* sr, int twl, int thl) {
-    const float4 d = *reinterpret_cast<const float4*>(r);
-    const TileRect t = rec_tiles(*r, twl, thl);
-    if (r->bx0 > r->bx1) return t;  // not rasterised; no shading inputs were written
-    sr->uz[0] = sr->uz[0] / d.y;
-    sr->vz[0] = sr->vz[0] / d.y;
-    sr->uz[1] = sr->uz[1] / d.z;
-    sr->vz[1] = sr->vz[1] / d.z;
-    sr->uz[2] = sr->uz[2] / d.w;
-    sr->vz[2] = sr->vz[2] / d.w;
-    if (!r->wide && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w)) {
+    const float4 d = gload(reinterpret_cast<const float4*>(r));
+    const uint4 p2 = gload(reinterpret_cast<const uint4*>(r) + 2);
+    const TileRect t = box_tiles(p2, twl, thl);
+    if (t.tx0 > t.tx1) return t;  // not rasterised; no shading inputs were written
+    // Pieces 0 and 1 of the shading record: uz[0..2], vz[0] | vz[1], vz[2], in[0], in[1].
+    static_assert(offsetof(ShadeRec, vz) == 12 && offsetof(ShadeRec, in) == 24,
+                  "shading record pieces");
+    float4 s0 = gload(reinterpret_cast<const float4*>(sr));
+    float4 s1 = gload(reinterpret_cast<const float4*>(sr) + 1);
+    s0.x = s0.x / d.y;  // uz[0]
+    s0.w = s0.w / d.y;  // vz[0]
+    s0.y = s0.y / d.z;  // uz[1]
+    s1.x = s1.x / d.z;  // vz[1]
+    s0.z = s0.z / d.w;  // uz[2]
+    s1.y = s1.y / d.w;  // vz[2]
+    gstore(reinterpret_cast<float4*>(sr), s0);
+    gstore(reinterpret_cast<float4*>(sr) + 1, s1);
+    if (!p2.z && rcp_range(d.x) && rcp_range(d.y) && rcp_range(d.z) && rcp_range(d.w)) {
         const float4 rc = make_float4(1.0f / d.x, 1.0f / d.y, 1.0f / d.z, 1.0f / d.w);
-        reinterpret_cast<float4*>(r)[1] = rc;
-        sr->rsf = __float_as_uint(rc.x);  // the shading pass reads it from here
+        gstore(reinterpret_cast<float4*>(r) + 1, rc);
+        gst(&sr->rsf, __float_as_uint(rc.x));  // the shading pass reads it from here
     }
     return t;
 }
@@ -895,15 +939,16 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds4, int& total
 // Clip-space corner k of a face: MVP * (x, y, z, w) (renderer.go:303-304,318-320).
 __device__ __forceinline__ float4 corner(const DevObject& o, int v) {
     const float* p = o.m.pos + 3 * (size_t)v;
-    float w = o.m.pw ? o.m.pw[v] : 1.0f;
-    return xform(o.mvp, p[0], p[1], p[2], w);
+    float w = o.m.pw ? gld(&o.m.pw[v]) : 1.0f;
+    return xform(o.mvp, gld(p), gld(p + 1), gld(p + 2), w);
 }
 
 // Gouraud intensity of normal n: World * normal, 4-D normalise (renderer.go:308-310,328-333).
 __device__ __forceinline__ float normal_intensity(const DevObject& o, const DevFrame& fr, int n) {
     const float* q = o.m.nrm + 3 * (size_t)n;
-    float w = o.m.nw ? o.m.nw[n] : 1.0f;
-    return lambert(xform(o.world, q[0], q[1], q[2], w), fr.light, fr.ambient, fr.diffuse);
+    float w = o.m.nw ? gld(&o.m.nw[n]) : 1.0f;
+    return lambert(xform(o.world, gld(q), gld(q + 1), gld(q + 2), w), fr.light, fr.ambient,
+                   fr.diffuse);
 }
 
 // LDS writes of one wave made visible to the wave's other lanes.
@@ -978,9 +1023,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int k = 0; k < kFacesPerLane; k++) {
             const int face = min(face0 + 64 * k, m.F - 1);
-            iv[k][0] = m.vi[3 * face + 0];
-            iv[k][1] = m.vi[3 * face + 1];
-            iv[k][2] = m.vi[3 * face + 2];
+            iv[k][0] = gld(&m.vi[3 * face + 0]);
+            iv[k][1] = gld(&m.vi[3 * face + 1]);
+            iv[k][2] = gld(&m.vi[3 * face + 2]);
         }
         float4 cv[kFacesPerLane][3];
 #pragma unroll
@@ -1062,7 +1107,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const uint32_t e = lds_sface[r0 + lane];
             const int sf = wg_in_obj * kFacesPerWg + (int)(e & 0xFFu);
             const int sp = (int)(e >> 8);
-            const int i0 = m.vi[3 * sf + 0], i1 = m.vi[3 * sf + 1], i2 = m.vi[3 * sf + 2];
+            const int i0 = gld(&m.vi[3 * sf + 0]), i1 = gld(&m.vi[3 * sf + 1]),
+                      i2 = gld(&m.vi[3 * sf + 2]);
             // The corners again (the same arithmetic as pass 1).
             const float4 w0 = corner(o, i0), w1 = corner(o, i1), w2 = corner(o, i2);
             c0 = CV{w0.x, w0.y, w0.z, w0.w, 0.f, 0.f, 0.f};
@@ -1073,16 +1119,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 if (o.gouraud) {
                     int n0 = i0, n1 = i1, n2 = i2;
                     if (m.ni) {
-                        n0 = m.ni[3 * sf + 0];
-                        n1 = m.ni[3 * sf + 1];
-                        n2 = m.ni[3 * sf + 2];
+                        n0 = gld(&m.ni[3 * sf + 0]);
+                        n1 = gld(&m.ni[3 * sf + 1]);
+                        n2 = gld(&m.ni[3 * sf + 2]);
                     }
                     in0 = normal_intensity(o, fr, n0);
                     in1 = normal_intensity(o, fr, n1);
                     in2 = normal_intensity(o, fr, n2);
                 } else {
                     const float* f = m.fn + 4 * (size_t)sf;
-                    float4 t = xform(o.world, f[0], f[1], f[2], f[3]);
+                    float4 t = xform(o.world, gld(f), gld(f + 1), gld(f + 2), gld(f + 3));
                     in0 = in1 = in2 = lambert(t, fr.light, fr.ambient, fr.diffuse);
                 }
             } else {
@@ -1092,22 +1138,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             c1.i = in1;
             c2.i = in2;
             if (m.uv_vert) {
-                c0.u = m.uv_vert[2 * i0];
-                c0.v = m.uv_vert[2 * i0 + 1];
-                c1.u = m.uv_vert[2 * i1];
-                c1.v = m.uv_vert[2 * i1 + 1];
-                c2.u = m.uv_vert[2 * i2];
-                c2.v = m.uv_vert[2 * i2 + 1];
+                c0.u = gld(&m.uv_vert[2 * i0]);
+                c0.v = gld(&m.uv_vert[2 * i0 + 1]);
+                c1.u = gld(&m.uv_vert[2 * i1]);
+                c1.v = gld(&m.uv_vert[2 * i1 + 1]);
+                c2.u = gld(&m.uv_vert[2 * i2]);
+                c2.v = gld(&m.uv_vert[2 * i2 + 1]);
             } else if (m.uv_face) {
                 const float* uv = m.uv_face + 6 * (size_t)sf;
-                c0.u = uv[0];
-                c0.v = uv[1];
-                c1.u = uv[2];
-                c1.v = uv[3];
-                c2.u = uv[4];
-                c2.v = uv[5];
+                c0.u = gld(uv);
+                c0.v = gld(uv + 1);
+                c1.u = gld(uv + 2);
+                c1.v = gld(uv + 3);
+                c2.u = gld(uv + 4);
+                c2.v = gld(uv + 5);
             }
-            if (flags & kShowTextures) tex = m.face_tex ? m.face_tex[sf] : m.mesh_tex;
+            if (flags & kShowTextures) tex = m.face_tex ? gld(&m.face_tex[sf]) : m.mesh_tex;
             if (sp == 6) {
                 // Every pass only rotates the triangle; after 6 passes it is
                 // unchanged (clipping.go:199-232); no clipping: as is.
@@ -1229,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_overlay(const DevObject* __restrict__ o
     const uint32_t cnt = wg_count[b];
     const uint32_t blk = (uint32_t)(o.seq_block + (b - o.face_block));  // in the frame
     const uint32_t slot0 = blk * kSlotsPerWg;
-    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? fr.rbase[blk] : 0u;
+    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? gld(&fr.rbase[blk]) : 0u;
     for (uint32_t i = threadIdx.x; i < cnt; i += 256) {
         const OverlayRec rec = fr.orec[block_rec(b, ovf, i)];
         // A frame with a line too long is counted on its own, besides the
@@ -1452,7 +1498,7 @@ __global__ __launch_bounds__(kFillThreads) __attribute__((amdgpu_waves_per_eu(8)
     // The block's slots start at slot0; its records are block_rec(b, ovf, i).
     const uint32_t blk = (uint32_t)(o.seq_block + (b - o.face_block));  // in the frame
     const uint32_t slot0 = blk * kSlotsPerWg;
-    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? fr.rbase[blk] : 0u;
+    const uint32_t ovf = cnt > (uint32_t)kRecsPerBlock ? gld(&fr.rbase[blk]) : 0u;
     // Tile ranges of the block's first kFillKeep * kFillThreads records stay in
     // registers from the finalising pass to the placing pass (the records are
     // read once); later records, if any, are read again in each pass.
@@ -1494,7 +1540,7 @@ __global__ __launch_bounds__(kFillThreads) __attribute__((amdgpu_waves_per_eu(8)
             if (i < cnt) fn(tr[k], i);
         }
         for (uint32_t i = threadIdx.x + kFillKeep * kFillThreads; i < cnt; i += kFillThreads)
-            fn(rec_tiles(fr.rrec[block_rec(b, ovf, i)], twl, thl), i);
+            fn(rec_tiles(&fr.rrec[block_rec(b, ovf, i)], twl, thl), i);
     };
     // The union of the ranges, by LDS atomics.
     __syncthreads();
@@ -1654,21 +1700,6 @@ static_assert(offsetof(RasterRec, bx0) == 32 && offsetof(RasterRec, by0) == 36 &
                   sizeof(RasterRec) == 80,
               "raster records are staged as five uint4 pieces");
 
-// Load of a 16-byte-aligned record through a global-address-space pointer
-// (global_load rather than flat, so LDS traffic is not waited on with it).
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-template <typename T>
-__device__ __forceinline__ T gload(const T* p) {
-    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "16-byte records only");
-    typedef const __attribute__((address_space(1))) u32x4* gptr;
-    const gptr src = (gptr)p;
-    u32x4 tmp[sizeof(T) / 16];
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(T) / 16); i++) tmp[i] = src[i];
-    T out;
-    __builtin_memcpy(&out, tmp, sizeof(T));
-    return out;
-}
 
 // Exact depth of `count` queued fragments of one wave (rasterizer.go:149-156),
 // one per lane, and the (depth, sequence) max into the tile key.
