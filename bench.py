@@ -4,10 +4,11 @@ Workload: the headline frame (displaced lat-long sphere of 200,000 triangles,
 16x16 RGBA8 texture, 1280x720, camera at r=1.2). Frame k rotates the object by
 0.01*k rad about Y, as the reference's demo loop does (main.go:229-233), so
 the frames of a launch are distinct work. A step is one batched launch of
---batch frames (256 by default, k = 0..255): the full path (vertex transform,
-clip, bin, raster, shade, colour + depth written to HBM) over 256 distinct
-frames, meshes and textures resident in HBM. --warmup steps run untimed first, then exactly --steps steps
-are timed between two device synchronisations.
+--batch frames (512 by default, k = 0..511): the full path (vertex transform,
+clip, bin, raster, shade, colour + depth written to HBM) over 512 distinct
+frames, meshes and textures resident in HBM. --warmup steps run untimed
+first, then exactly --steps steps are timed between two device
+synchronisations.
 
 Forms timed: "graph" (each batch's launches captured in HIP graphs and
 replayed, two batches alternating on the context's two pipelines), "stream"
@@ -41,7 +42,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=64,
                     help="timed steps; a step is one batched launch of --batch frames")
     ap.add_argument("--warmup", type=int, default=16, help="untimed steps before timing")
-    ap.add_argument("--batch", type=int, default=256, help="frames per batched launch")
+    ap.add_argument("--batch", type=int, default=512, help="frames per batched launch")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
