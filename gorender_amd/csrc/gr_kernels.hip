@@ -73,9 +73,10 @@ __device__ __forceinline__ int64_t wmul(int64_t a, int64_t b) {
     return (int64_t)((uint64_t)a * (uint64_t)b);
 }
 
-// Scalar load and store through a global-address-space pointer: the pointers
-// of DevFrame are generic, and a generic (flat) access also counts against
-// the LDS wait counter, so a later LDS wait would wait for it too.
+// Load and store of one value through a global-address-space pointer: the
+// pointers of DevFrame and DevMesh are generic, and a generic (flat) access
+// also counts against the LDS wait counter, so a later LDS wait would wait for
+// it too.
 template <typename T>
 __device__ __forceinline__ T gld(const T* p) {
     return *(const __attribute__((address_space(1))) T*)p;
@@ -834,9 +835,8 @@ struct TileRect {
     int tx0, tx1, ty0, ty1;
 };
 
-// Tile range of a raster record (empty: tx0 > tx1).
-// Tile range of a record's box from its piece 2 (bx0 | bx1 << 16, by0 | by1 << 16,
-// wide, A01); an empty box gives an empty range.
+// Tile range of a raster record's box (empty: tx0 > tx1), from the record's
+// piece 2 (bx0 | bx1 << 16, by0 | by1 << 16, wide, A01).
 __device__ __forceinline__ TileRect box_tiles(const uint4& p2, int twl, int thl) {
     const int bx0 = (int)(int16_t)(p2.x & 0xFFFFu), bx1 = (int)(int16_t)(p2.x >> 16);
     const int by0 = (int)(int16_t)(p2.y & 0xFFFFu), by1 = (int)(int16_t)(p2.y >> 16);
