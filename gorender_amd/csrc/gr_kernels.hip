@@ -40,9 +40,9 @@ __device__ __forceinline__ int64_t go_int(float x) {
 }
 
 // uint8(float32): CVTTSS2SL then the low byte; NaN/out of int32 range -> 0.
+// One compare: -2^31 itself converts to INT32_MIN, whose low byte is 0 too.
 __device__ __forceinline__ uint32_t go_uint8(float x) {
-    if (!(x >= -2147483648.0f && x < 2147483648.0f)) return 0u;
-    return (uint32_t)(int32_t)x & 0xFFu;
+    return __builtin_fabsf(x) < 2147483648.0f ? (uint32_t)(int32_t)x & 0xFFu : 0u;
 }
 
 __device__ __forceinline__ bool f_signbit(float a) { return (__float_as_uint(a) >> 31) != 0; }
@@ -2186,7 +2186,7 @@ __device__ __forceinline__ void raster_tile(
 // XCD-aware tile order. Workgroup h runs on XCD h mod 8; each XCD takes runs
 // of kTileRun consecutive tiles (runs x, x + 8, x + 16, ... for XCD x), so
 // neighbouring tiles, which share triangle records, read them through one L2.
-constexpr int kTileRun = 64;
+constexpr int kTileRun = 128;
 __device__ __forceinline__ int xcd_tile(int h) {
     const int x = h & 7, j = h >> 3;
     return ((j / kTileRun) * 8 + x) * kTileRun + j % kTileRun;
