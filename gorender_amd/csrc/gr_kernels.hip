@@ -1692,6 +1692,16 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// A*x + B*y + C in wrapping 32-bit arithmetic, for A, B, x, y that fit 24
+// bits signed, as two multiply-adds (v_mad_i32_i24). The empty asm keeps the
+// partial sum whole, so that the compiler does not turn the two products and
+// two additions into two multiplies and a three-operand add.
+__device__ __forceinline__ uint32_t edge24(int A, int x, int B, int y, uint32_t C) {
+    uint32_t u = (uint32_t)__mul24(B, y) + C;
+    asm("" : "+v"(u));
+    return (uint32_t)__mul24(A, x) + u;
+}
+
 // The raster stage reads a staged RasterRec as five 16-byte pieces: divisors,
 // reciprocals, then (box x, box y, wide, A01), (B01, C01, A12, B12),
 // (C12, A20, B20, C20).
@@ -1863,9 +1873,9 @@ __device__ __forceinline__ unsigned long long raster_list(
             // edge value fits int32 (|x|, |y| < 2^14).
             const int A0 = (int)w3.x, A1 = (int)w3.y, A2 = (int)w3.z;
             const int B0 = (int)w3.w, B1 = (int)w4.x, B2 = (int)w4.y;
-            int f0 = __mul24(A0, lx) + __mul24(B0, ly) + (int)w2.y;
-            int f1 = __mul24(A1, lx) + __mul24(B1, ly) + (int)w2.z;
-            int f2 = __mul24(A2, lx) + __mul24(B2, ly) + (int)w2.w;
+            int f0 = (int)edge24(A0, lx, B0, ly, w2.y);
+            int f1 = (int)edge24(A1, lx, B1, ly, w2.z);
+            int f2 = (int)edge24(A2, lx, B2, ly, w2.w);
             const int sy0 = (int)((uint32_t)B0 << (6 - l2));  // B * rows
             const int sy1 = (int)((uint32_t)B1 << (6 - l2));
             const int sy2 = (int)((uint32_t)B2 << (6 - l2));
@@ -1970,10 +1980,8 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             // The int32 edge values are formed before the wide test; a wide
             // record replaces them (its int32 values are never used, and the
             // sums wrap as unsigned).
-            float f12f = (float)(int32_t)((uint32_t)__mul24((int)e3.z, x) +
-                                          (uint32_t)__mul24((int)e3.w, y) + e4.x);
-            float f20f = (float)(int32_t)((uint32_t)__mul24((int)e4.y, x) +
-                                          (uint32_t)__mul24((int)e4.z, y) + e4.w);
+            float f12f = (float)(int32_t)edge24((int)e3.z, x, (int)e3.w, y, e4.x);
+            float f20f = (float)(int32_t)edge24((int)e4.y, x, (int)e4.z, y, e4.w);
             if (s.rsf == kShadeWide) {
                 int64_t f[3];
                 wide_edges(&fr.wrec[ri], x, y, f);
