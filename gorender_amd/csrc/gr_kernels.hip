@@ -114,6 +114,16 @@ __device__ __forceinline__ void gstore(T* p, const T& v) {
     for (int i = 0; i < (int)(sizeof(T) / 16); i++) dst[i] = tmp[i];
 }
 
+// A*x + B*y + C in wrapping 32-bit arithmetic, for A, B, x, y that fit 24
+// bits signed, as two multiply-adds (v_mad_i32_i24). The empty asm keeps the
+// partial sum whole, so that the compiler does not turn the two products and
+// two additions into two multiplies and a three-operand add.
+__device__ __forceinline__ uint32_t edge24(int A, int x, int B, int y, uint32_t C) {
+    uint32_t u = (uint32_t)__mul24(B, y) + C;
+    asm("" : "+v"(u));
+    return (uint32_t)__mul24(A, x) + u;
+}
+
 // ---------------------------------------------------------------------------
 // Exact float32 division from a correctly rounded reciprocal
 // ---------------------------------------------------------------------------
@@ -714,9 +724,9 @@ __device__ uint32_t emit_triangle(const DevFrame& fr, const CV& a, const CV& b, 
                     for (int k = 0; k < 4; k++) {
                         const int32_t x = (int32_t)min(mnx + (k & 1), mxx);
                         const int32_t y = (int32_t)min(mny + (k >> 1), mxy);
-                        const int32_t f0 = __mul24(rr.e[0], x) + __mul24(rr.e[1], y) + rr.e[2];
-                        const int32_t f1 = __mul24(rr.e[3], x) + __mul24(rr.e[4], y) + rr.e[5];
-                        const int32_t f2 = __mul24(rr.e[6], x) + __mul24(rr.e[7], y) + rr.e[8];
+                        const int32_t f0 = (int32_t)edge24(rr.e[0], x, rr.e[1], y, rr.e[2]);
+                        const int32_t f1 = (int32_t)edge24(rr.e[3], x, rr.e[4], y, rr.e[5]);
+                        const int32_t f2 = (int32_t)edge24(rr.e[6], x, rr.e[7], y, rr.e[8]);
                         any |= f0 & f1 & f2;
                     }
                     if (any >= 0) {
@@ -1616,14 +1626,22 @@ struct Bary {
     float alpha, beta, gamma, zrec;
 };
 
+// The reciprocal path of bary(), for a triangle whose reciprocals are set.
+__device__ __forceinline__ Bary bary_rcp(float f12f, float f20f, const float4& d,
+                                         const float4& rc) {
+    Bary b;
+    b.alpha = div_rcp(f12f, d.x, rc.x);
+    b.beta = div_rcp(f20f, d.x, rc.x);
+    b.gamma = 1.0f - b.alpha - b.beta;
+    b.zrec = -(div_rcp(b.alpha, d.y, rc.y) + div_rcp(b.beta, d.z, rc.z) +
+               div_rcp(b.gamma, d.w, rc.w));
+    return b;
+}
+
 __device__ __forceinline__ Bary bary(float f12f, float f20f, const float4& d, const float4& rc) {
     Bary b;
     if (rc.x != 0.f) {
-        b.alpha = div_rcp(f12f, d.x, rc.x);
-        b.beta = div_rcp(f20f, d.x, rc.x);
-        b.gamma = 1.0f - b.alpha - b.beta;
-        b.zrec = -(div_rcp(b.alpha, d.y, rc.y) + div_rcp(b.beta, d.z, rc.z) +
-                   div_rcp(b.gamma, d.w, rc.w));
+        b = bary_rcp(f12f, f20f, d, rc);
     } else {
         b.alpha = f12f / d.x;
         b.beta = f20f / d.x;
@@ -1692,15 +1710,6 @@ __device__ __forceinline__ uint32_t color_intensity(uint32_t c, float k) {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// A*x + B*y + C in wrapping 32-bit arithmetic, for A, B, x, y that fit 24
-// bits signed, as two multiply-adds (v_mad_i32_i24). The empty asm keeps the
-// partial sum whole, so that the compiler does not turn the two products and
-// two additions into two multiplies and a three-operand add.
-__device__ __forceinline__ uint32_t edge24(int A, int x, int B, int y, uint32_t C) {
-    uint32_t u = (uint32_t)__mul24(B, y) + C;
-    asm("" : "+v"(u));
-    return (uint32_t)__mul24(A, x) + u;
-}
 
 // The raster stage reads a staged RasterRec as five 16-byte pieces: divisors,
 // reciprocals, then (box x, box y, wide, A01), (B01, C01, A12, B12),
@@ -1727,11 +1736,14 @@ __device__ __forceinline__ void frag_depth(const uint32_t (*q)[128], uint32_t he
         const int32_t f12 = (int32_t)q[1][k], f20 = (int32_t)q[2][k];
         const uint32_t i = pk >> 12, px_idx = pk & 0xFFFu;
         const uint4 d = lrec[i * 5], rc = lrec[i * 5 + 1];
-        const Bary b = bary((float)f12, (float)f20,
-                            make_float4(__uint_as_float(d.x), __uint_as_float(d.y),
-                                        __uint_as_float(d.z), __uint_as_float(d.w)),
-                            make_float4(__uint_as_float(rc.x), __uint_as_float(rc.y),
-                                        __uint_as_float(rc.z), __uint_as_float(rc.w)));
+        const float4 df = make_float4(__uint_as_float(d.x), __uint_as_float(d.y),
+                                      __uint_as_float(d.z), __uint_as_float(d.w));
+        const float4 rf = make_float4(__uint_as_float(rc.x), __uint_as_float(rc.y),
+                                      __uint_as_float(rc.z), __uint_as_float(rc.w));
+        // Almost every triangle has reciprocals: when all of the wave's do, the
+        // branch is taken once for the wave instead of per lane.
+        const Bary b = __ballot(rf.x == 0.f) == 0ull ? bary_rcp((float)f12, (float)f20, df, rf)
+                                                      : bary((float)f12, (float)f20, df, rf);
         if (b.zrec >= -1.0f) atomicMax(&key[px_idx], depth_key(b.zrec, lslot[i] + 1u));
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
