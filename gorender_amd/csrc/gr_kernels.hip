@@ -1994,34 +1994,45 @@ __device__ __forceinline__ unsigned long long shade_tile(const DevFrame& fr, int
             // sums wrap as unsigned).
             float f12f = (float)(int32_t)edge24((int)e3.z, x, (int)e3.w, y, e4.x);
             float f20f = (float)(int32_t)edge24((int)e4.y, x, (int)e4.z, y, e4.w);
-            if (s.rsf == kShadeWide) {
-                int64_t f[3];
-                wide_edges(&fr.wrec[ri], x, y, f);
-                f12f = (float)f[1];
-                f20f = (float)f[2];
-            }
             // The key holds zRec's bits (with -0 folded into +0): only a zero
             // zRec is recomputed; alpha and beta are as in bary().
             const uint32_t ord = (uint32_t)(k >> 32);
             const float zkey = __uint_as_float((ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord);
+            // rsf is 0 or kShadeWide (IEEE division) unless the raster record
+            // holds reciprocals.
+            const bool common = zkey != 0.0f && s.rsf != 0u && s.rsf != kShadeWide;
             Bary b;
-            if (zkey != 0.0f) {
-                // rsf is 0 or kShadeWide (IEEE division) unless the raster
-                // record holds reciprocals.
+            if (__ballot(!common) == 0ull) {
+                // Every pixel of the wave: int32 edge values, reciprocals, and
+                // zRec from the key (one branch for the wave, not per lane).
                 const float rsf = __uint_as_float(s.rsf);
-                if (s.rsf != 0u && s.rsf != kShadeWide) {
-                    b.alpha = div_rcp(f12f, s.sf, rsf);
-                    b.beta = div_rcp(f20f, s.sf, rsf);
-                } else {
-                    b.alpha = f12f / s.sf;
-                    b.beta = f20f / s.sf;
-                }
+                b.alpha = div_rcp(f12f, s.sf, rsf);
+                b.beta = div_rcp(f20f, s.sf, rsf);
                 b.gamma = 1.0f - b.alpha - b.beta;
                 b.zrec = zkey;
             } else {
-                const float4 d = gload(reinterpret_cast<const float4*>(rq)),
-                             rc = gload(reinterpret_cast<const float4*>(rq) + 1);
-                b = bary(f12f, f20f, d, rc);
+                if (s.rsf == kShadeWide) {
+                    int64_t f[3];
+                    wide_edges(&fr.wrec[ri], x, y, f);
+                    f12f = (float)f[1];
+                    f20f = (float)f[2];
+                }
+                if (zkey != 0.0f) {
+                    const float rsf = __uint_as_float(s.rsf);
+                    if (s.rsf != 0u && s.rsf != kShadeWide) {
+                        b.alpha = div_rcp(f12f, s.sf, rsf);
+                        b.beta = div_rcp(f20f, s.sf, rsf);
+                    } else {
+                        b.alpha = f12f / s.sf;
+                        b.beta = f20f / s.sf;
+                    }
+                    b.gamma = 1.0f - b.alpha - b.beta;
+                    b.zrec = zkey;
+                } else {
+                    const float4 d = gload(reinterpret_cast<const float4*>(rq)),
+                                 rc = gload(reinterpret_cast<const float4*>(rq) + 1);
+                    b = bary(f12f, f20f, d, rc);
+                }
             }
             // rasterizer.go:158-170
             float u = (b.alpha * s.uz[0] + b.beta * s.uz[1] + b.gamma * s.uz[2]) / b.zrec;
